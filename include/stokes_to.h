@@ -1,0 +1,166 @@
+/*
+ * stokes_to.h -- C ABI of libstokes_hip.so, the MI355X (gfx950) implementation of
+ * the Stokes/Brinkman topology-optimization hot path of jyl-pages/Stokes_TO
+ * (app `fluid_topo`).
+ *
+ * Conventions
+ *  - Every function returns ST_OK (0) or a negative ST_E* code; nothing exits the
+ *    process and no C++ exception crosses the ABI.  st_last_error() gives text.
+ *  - Host arrays use the ABI layout of the reference's host fields
+ *    (SX/reservoir/Field.h:67-72): C-order (i, j, k) with k fastest.
+ *      cells  : n*n*n            (2D: n*n)
+ *      faces d: dims n+1 along d, n across (2D: x, y only)
+ *    A dof vector / descriptor array is [cells | faces_x | faces_y | faces_z],
+ *    the order of the reference's solver vectors (ST/FluidSteadyState3D.cpp:101-105).
+ *  - Vectors are FP64 at the ABI; the preconditioner precision is internal.
+ *  - A context owns one device, one stream and all device state; it is not
+ *    thread-safe (like the reference's LinearMapping objects, which keep
+ *    internal scratch: CX/LinearMapping.h:4-14).
+ *  - Fixed DOFs (Dirichlet faces) have identity rows ("definite fix",
+ *    ST/StokeFlowMapping3D.cpp:301-305); face vol equals the free mask
+ *    (ST/StokeFlowDescriptor3D.cu:37-50).
+ */
+#ifndef STOKES_TO_H
+#define STOKES_TO_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct st_ctx st_ctx;
+
+enum {
+    ST_OK = 0,
+    ST_EINVAL = -1,     /* bad argument or unsupported configuration */
+    ST_EHIP = -2,       /* HIP runtime failure (allocation, launch) */
+    ST_ENOCONV = -3,    /* Krylov iteration cap reached before rtol */
+    ST_ESINGULAR = -4,  /* coarsest-level matrix is singular (closed domain without gauge) */
+    ST_ESTATE = -5,     /* call order: geometry / penalty / rhs missing */
+    ST_ENAN = -6        /* NaN or Inf in an input or in the residual */
+};
+
+enum { ST_FP64 = 0, ST_FP32 = 1, ST_BF16 = 2 };        /* preconditioner precision */
+enum { ST_INIT_ZERO = 0, ST_INIT_KEEP = 1, ST_INIT_HOST = 2, ST_INIT_LEARNED = 3 };
+
+typedef struct {
+    int dim;             /* 2 or 3 */
+    int n;               /* cells per axis; n = 4 * 2^k */
+    /* design -> Brinkman penalty: alpha(r) = alpha_max + (alpha_min - alpha_max) r(1+q)/(r+q)
+       on r = rho^power, times alpha_scale (FT/FluidTopoOpt3D.cpp:110-123, :145-159) */
+    double alpha_max, alpha_min, q, power, alpha_scale;
+    int precision;       /* ST_FP64 / ST_FP32 / ST_BF16 multigrid preconditioner */
+    double rtol;         /* stop on the true FP64 residual ||b-Ax||/||b|| <= rtol */
+    int max_iter;        /* total FGMRES iterations */
+    int restart;         /* FGMRES restart length (<= 32) */
+    int nu_pre, nu_post; /* smoothing sweeps (each = both colours) */
+    double omega;        /* smoother damping */
+    int closed;          /* add the all-ones pressure gauge on the coarsest level */
+    double filter_radius;/* density filter radius in cells (<= 1 disables) */
+} st_params;
+
+typedef struct {
+    int iters;           /* FGMRES iterations (= V-cycles) */
+    int converged;       /* 1 if relres <= rtol */
+    double relres;       /* true FP64 relative residual at exit */
+    double relres0;      /* true relative residual of the initial guess */
+    double ms;           /* wall time of the solve incl. hierarchy setup (hipEvents) */
+    double setup_ms;     /* hierarchy setup part (coarsening + coarsest inverse) */
+    double init_ms;      /* initial-guess part (learned predictor inference) */
+} st_stats;
+
+/* defaults: the configs' material model (alpha_max 1e4, q 0.1, power 1, scale 1/n^2),
+   FP64, rtol 1e-8, FGMRES(30), V(2,2), omega 1, filter radius 1.5 */
+void st_default_params(st_params *p, int dim, int n);
+
+/* replaces FluidSteadyState3D::init (ST/FluidSteadyState3D.cpp:12-90): builds the
+   multigrid hierarchy and all device buffers */
+int st_create(st_ctx **out, const st_params *p);
+void st_destroy(st_ctx *c);
+const char *st_last_error(const st_ctx *c);
+int st_get_params(const st_ctx *c, st_params *p);
+/* number of dofs of the ABI vector and of the design (cells) */
+long st_num_dofs(const st_ctx *c);
+long st_num_cells(const st_ctx *c);
+
+/* replaces FluidSteadyState3D::init_boundary (ST/FluidSteadyState3D.cpp:92-114):
+   fixed[cells|faces], 1 = fixed (Dirichlet) DOF; vol is the free mask */
+int st_set_geometry(st_ctx *c, const uint8_t *fixed);
+/* replaces FluidSteadyState3D::update_penalty (ST/FluidSteadyState3D.cpp:130-150):
+   pen[cells|faces]; cell penalties must be zero */
+int st_set_penalty(st_ctx *c, const double *pen);
+/* replaces the host loop of FluidTopoOpt3D::Update_Fluid (FT/FluidTopoOpt3D.cpp:145-159):
+   rho[cells] -> face alpha on the device */
+int st_set_design(st_ctx *c, const double *rho);
+int st_get_design(st_ctx *c, double *rho);
+/* device-pointer variant of st_set_design (rho in device memory, ABI cell order) */
+int st_set_design_device(st_ctx *c, const double *d_rho);
+/* replaces FluidSteadyState3D::update_b (ST/FluidSteadyState3D.cpp:116-127):
+   body[dofs] (nullable) are source terms on free rows; dirichlet[dofs] (nullable)
+   are values on fixed faces, lifted into the right-hand side */
+int st_set_rhs(st_ctx *c, const double *body, const double *dirichlet);
+/* replaces FluidSteadyState3D::solve + GMRES::Solve (ST/FluidSteadyState3D.cpp:152-170,
+   CX/GMRES.cpp:20-110): FGMRES right-preconditioned by one V-cycle, stopped on the
+   true FP64 residual.  init: ST_INIT_ZERO (Dirichlet values, zero elsewhere),
+   ST_INIT_KEEP (the device solution of the previous solve), ST_INIT_HOST (x0),
+   ST_INIT_LEARNED (the learned predictor).  x_out (nullable) receives the solution. */
+int st_solve(st_ctx *c, int init, const double *x0, double *x_out, st_stats *stats);
+int st_get_solution(st_ctx *c, double *x);
+/* replaces StokeFlowMapping3DFixed::applyMapping through LinearMapping
+   (ST/StokeFlowMapping3D.cu:176-306, CX/LinearMapping.h:4-14): y = A x, host arrays */
+int st_apply(st_ctx *c, const double *x, double *y);
+/* device-pointer variant (ABI layout in device memory, on the context's stream) */
+int st_apply_device(st_ctx *c, const double *d_x, double *d_y);
+/* replaces FluidEnergy3D::compute_energy (FT/FluidEnergy3D.cpp:94-109): J = x^T G x / 2 */
+int st_objective(st_ctx *c, const double *x, double *J);
+/* replaces FluidEnergy3D::compute_gradient_x (FT/FluidEnergy3D.cpp:71-81): gx = G x */
+int st_energy_grad(st_ctx *c, const double *x, double *gx);
+/* replaces the sensitivity loop of FluidTopoOpt3D::Update_Grad (FT/FluidTopoOpt3D.cpp:170-198):
+   dj[cells] = dJ/drho from forward x and adjoint lam (A lam = G x) */
+int st_sensitivity(st_ctx *c, const double *rho, const double *x, const double *lam, double *dj);
+/* density filter (no counterpart in the reference, which feeds rho to alpha directly,
+   FT/FluidTopoOpt3D.cpp:145-168): out = H in / (H 1), or its transpose */
+int st_filter(st_ctx *c, const double *in, double *out, int transpose);
+/* optimality-criteria design update with bisection on the volume multiplier, in the
+   slot of OptimizerMMA::Optimize_MMA (SX/optimization/OptimizerMma.h:138-170);
+   volume measured on the filtered density */
+int st_oc_update(st_ctx *c, const double *rho, const double *dc, double volfrac, double move, double eta,
+                 double rho_min, double *rho_new, double *change);
+/* one full design iteration on the device, the body of Optimize_MMA's loop
+   (FT/FluidTopoOpt3D.cpp:4-17): filter, design->alpha, forward solve, objective,
+   adjoint solve, sensitivity, filter transpose, OC update of the device design */
+int st_to_step(st_ctx *c, double volfrac, double move, double eta, double rho_min, double *J, double *vol,
+               double *change, st_stats *fwd, st_stats *adj);
+
+/* ---- kernel timing (HIP events around each launch on the context stream) ---- */
+enum { ST_PROF_SMOOTH = 0, ST_PROF_APPLY = 1, ST_PROF_RESIDUAL = 2, ST_PROF_NKIND = 3 };
+/* on != 0 starts recording the finest-level launches of each kind; 0 stops */
+int st_profile(st_ctx *c, int on);
+/* total device ms and launch count recorded for `kind` since st_profile(c, 1);
+   bytes = algorithmic HBM bytes of one launch of that kind (FP64 unless the MG
+   precision applies) */
+int st_profile_read(st_ctx *c, int kind, double *total_ms, long *launches, double *bytes_per_launch);
+
+/* ---- learned initial guess (no counterpart in the reference) ---- */
+/* train the conv-net predictor on the (design, solution) pairs recorded by previous
+   st_to_step / st_record_sample calls; returns the final training loss */
+int st_record_sample(st_ctx *c);
+int st_net_train(st_ctx *c, int epochs, double lr, double *loss);
+
+/* ---- multigrid internals exposed for component parity tests ---- */
+int st_mg_levels(const st_ctx *c);
+int st_mg_level_n(const st_ctx *c, int level);
+/* one V-cycle z = M b (host arrays, ABI layout of level 0) */
+int st_mg_vcycle(st_ctx *c, const double *b, double *z);
+/* level descriptor: fixed flags [cells|faces] and face alpha (FP64 hierarchy) */
+int st_mg_level_desc(st_ctx *c, int level, uint8_t *fixed, double *alpha_faces);
+/* one smoother colour step on `level`: xout = S_color(b, xin) */
+int st_mg_smooth(st_ctx *c, int level, int color, const double *b, const double *xin, double *xout);
+/* bc = R rf (fine level -> level+1), xf_out = xf + P ec */
+int st_mg_restrict(st_ctx *c, int level, const double *rf, double *bc);
+int st_mg_prolong(st_ctx *c, int level, const double *ec, const double *xf, double *xf_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,305 @@
+"""CPU FP64 oracle (ctypes wrapper over oracle/liborc.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker and the CPU baseline.  The product
+package (stokes_to_amd) never imports it.
+
+Arrays use the ABI layout: C-order (i, j, k), k fastest; a dof vector is
+[p | u_x | u_y | u_z] and a descriptor array uses the same [cells | faces] order
+(ST/StokeFlowDescriptor3D.cu:6-50).
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "liborc.so")
+_lib = None
+
+D = ctypes.POINTER(ctypes.c_double)
+U8 = ctypes.POINTER(ctypes.c_uint8)
+I = ctypes.c_int
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", _HERE])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            build()
+        _lib = ctypes.CDLL(_LIB)
+        L = _lib
+        L.orc_dof.restype = ctypes.c_long
+        L.orc_dof.argtypes = [I, I, I, I, ctypes.POINTER(ctypes.c_long)]
+        L.orc_edge_count.restype = ctypes.c_long
+        for nm in ("orc_d2", "orc_cod0", "orc_cod1", "orc_d1"):
+            getattr(L, nm).argtypes = [I, I, I, I, D, D]
+        for nm in ("orc_apply", "orc_energy_op"):
+            getattr(L, nm).argtypes = [I, I, I, I, U8, D, D, D, D]
+        L.orc_apply_ref2d.argtypes = [I, I, U8, D, D, D, D]
+        L.orc_energy_op_ref2d.argtypes = [I, I, U8, D, D, D]
+        L.orc_lift_rhs.argtypes = [I, I, I, I, U8, D, D]
+        L.orc_alpha.restype = ctypes.c_double
+        L.orc_alpha.argtypes = [ctypes.c_double, D]
+        L.orc_dalpha.restype = ctypes.c_double
+        L.orc_dalpha.argtypes = [ctypes.c_double, D]
+        L.orc_design_to_alpha.argtypes = [I, I, I, I, D, D, U8, D]
+        L.orc_sensitivity.argtypes = [I, I, I, I, D, D, U8, D, D, D]
+        L.orc_objective.restype = ctypes.c_double
+        L.orc_objective.argtypes = [I, I, I, I, U8, D, D, D]
+        L.orc_filter.argtypes = [I, I, I, I, ctypes.c_double, D, D, I]
+        L.orc_oc_update.restype = I
+        L.orc_oc_update.argtypes = [I, I, I, I, ctypes.c_double, D, D, ctypes.c_double,
+                                    ctypes.c_double, ctypes.c_double, ctypes.c_double, D, D]
+        L.orc_mg_create.restype = ctypes.c_void_p
+        L.orc_mg_create.argtypes = [I, I, I, I, U8, D, D]
+        L.orc_mg_free.argtypes = [ctypes.c_void_p]
+        L.orc_mg_levels.restype = I
+        L.orc_mg_levels.argtypes = [ctypes.c_void_p]
+        L.orc_mg_level_dims.argtypes = [ctypes.c_void_p, I, ctypes.POINTER(I)]
+        L.orc_mg_level_desc.argtypes = [ctypes.c_void_p, I, U8, D]
+        L.orc_mg_vcycle.argtypes = [ctypes.c_void_p, D, D]
+        L.orc_mg_apply.argtypes = [ctypes.c_void_p, I, D, D]
+        L.orc_smooth_color.argtypes = [I, I, I, I, U8, D, D, D, D, I, ctypes.c_double]
+        L.orc_restrict.argtypes = [I, I, I, I, U8, I, D, D]
+        L.orc_prolong_add.argtypes = [I, I, I, I, U8, I, D, D]
+        L.orc_coarsen.argtypes = [I, I, I, I, U8, D, U8, D]
+        L.orc_fgmres.restype = I
+        L.orc_fgmres.argtypes = [ctypes.c_void_p, D, D, ctypes.c_double, I, I, D, D]
+    return _lib
+
+
+def _d(a):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(D)
+
+
+def _u(a):
+    assert a.dtype == np.uint8 and a.flags.c_contiguous
+    return a.ctypes.data_as(U8)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _b(a):
+    return np.ascontiguousarray(a, dtype=np.uint8)
+
+
+def sizes(dim, nx, ny, nz):
+    s = (ctypes.c_long * 4)()
+    n = lib().orc_dof(dim, nx, ny, nz, s)
+    return n, list(s)
+
+
+def edge_count(dim, nx, ny, nz):
+    return lib().orc_edge_count(dim, nx, ny, nz)
+
+
+def d2(dim, shape, faces):
+    nx, ny, nz = shape
+    n, s = sizes(dim, nx, ny, nz)
+    out = np.zeros(s[0])
+    lib().orc_d2(dim, nx, ny, nz, _d(_f64(faces)), _d(out))
+    return out
+
+
+def cod0(dim, shape, cells):
+    nx, ny, nz = shape
+    n, s = sizes(dim, nx, ny, nz)
+    out = np.zeros(n - s[0])
+    lib().orc_cod0(dim, nx, ny, nz, _d(_f64(cells)), _d(out))
+    return out
+
+
+def cod1(dim, shape, faces):
+    nx, ny, nz = shape
+    out = np.zeros(edge_count(dim, nx, ny, nz))
+    lib().orc_cod1(dim, nx, ny, nz, _d(_f64(faces)), _d(out))
+    return out
+
+
+def d1(dim, shape, edges):
+    nx, ny, nz = shape
+    n, s = sizes(dim, nx, ny, nz)
+    out = np.zeros(n - s[0])
+    lib().orc_d1(dim, nx, ny, nz, _d(_f64(edges)), _d(out))
+    return out
+
+
+def apply(dim, shape, fixed, vol, pen, x):
+    nx, ny, nz = shape
+    y = np.zeros_like(_f64(x))
+    lib().orc_apply(dim, nx, ny, nz, _u(_b(fixed)), _d(_f64(vol)), _d(_f64(pen)), _d(_f64(x)), _d(y))
+    return y
+
+
+def apply_ref2d(shape, fixed, vol, pen, x):
+    nx, ny = shape[:2]
+    y = np.zeros_like(_f64(x))
+    lib().orc_apply_ref2d(nx, ny, _u(_b(fixed)), _d(_f64(vol)), _d(_f64(pen)), _d(_f64(x)), _d(y))
+    return y
+
+
+def energy_op(dim, shape, fixed, vol, pen, x):
+    nx, ny, nz = shape
+    y = np.zeros_like(_f64(x))
+    lib().orc_energy_op(dim, nx, ny, nz, _u(_b(fixed)), _d(_f64(vol)), _d(_f64(pen)), _d(_f64(x)), _d(y))
+    return y
+
+
+def energy_op_ref2d(shape, fixed, pen, x):
+    nx, ny = shape[:2]
+    y = np.zeros_like(_f64(x))
+    lib().orc_energy_op_ref2d(nx, ny, _u(_b(fixed)), _d(_f64(pen)), _d(_f64(x)), _d(y))
+    return y
+
+
+def objective(dim, shape, fixed, vol, pen, x):
+    nx, ny, nz = shape
+    return lib().orc_objective(dim, nx, ny, nz, _u(_b(fixed)), _d(_f64(vol)), _d(_f64(pen)), _d(_f64(x)))
+
+
+def lift_rhs(dim, shape, fixed, xd):
+    nx, ny, nz = shape
+    b = np.zeros_like(_f64(xd))
+    lib().orc_lift_rhs(dim, nx, ny, nz, _u(_b(fixed)), _d(_f64(xd)), _d(b))
+    return b
+
+
+def alpha_params(alpha_max, alpha_min, q, power, scale):
+    return np.array([alpha_max, alpha_min, q, power, scale], dtype=np.float64)
+
+
+def alpha(rho, am):
+    return lib().orc_alpha(float(rho), _d(_f64(am)))
+
+
+def dalpha(rho, am):
+    return lib().orc_dalpha(float(rho), _d(_f64(am)))
+
+
+def design_to_alpha(dim, shape, rho, am, face_fixed):
+    nx, ny, nz = shape
+    n, s = sizes(dim, nx, ny, nz)
+    out = np.zeros(n - s[0])
+    lib().orc_design_to_alpha(dim, nx, ny, nz, _d(_f64(rho)), _d(_f64(am)), _u(_b(face_fixed)), _d(out))
+    return out
+
+
+def sensitivity(dim, shape, rho, am, face_fixed, u, lam):
+    nx, ny, nz = shape
+    n, s = sizes(dim, nx, ny, nz)
+    out = np.zeros(s[0])
+    lib().orc_sensitivity(dim, nx, ny, nz, _d(_f64(rho)), _d(_f64(am)), _u(_b(face_fixed)),
+                          _d(_f64(u)), _d(_f64(lam)), _d(out))
+    return out
+
+
+def filt(dim, shape, radius, x, transpose=False):
+    nx, ny, nz = shape
+    out = np.zeros_like(_f64(x))
+    lib().orc_filter(dim, nx, ny, nz, float(radius), _d(_f64(x)), _d(out), int(transpose))
+    return out
+
+
+def oc_update(dim, shape, radius, rho, dc, volfrac, move=0.2, eta=0.5, rho_min=1e-3):
+    nx, ny, nz = shape
+    out = np.zeros_like(_f64(rho))
+    ch = ctypes.c_double(0)
+    lib().orc_oc_update(dim, nx, ny, nz, float(radius), _d(_f64(rho)), _d(_f64(dc)), float(volfrac),
+                        float(move), float(eta), float(rho_min), _d(out), ctypes.byref(ch))
+    return out, ch.value
+
+
+def smooth_color(dim, shape, fixed, pen, b, xin, color, omega):
+    nx, ny, nz = shape
+    out = np.zeros_like(_f64(xin))
+    lib().orc_smooth_color(dim, nx, ny, nz, _u(_b(fixed)), _d(_f64(pen)), _d(_f64(b)), _d(_f64(xin)),
+                           _d(out), int(color), float(omega))
+    return out
+
+
+def restrict(dim, fshape, cfixed, tang, rf):
+    nx, ny, nz = fshape
+    cs = (nx // 2, ny // 2, nz // 2 if dim == 3 else 1)
+    n, _ = sizes(dim, *cs)
+    out = np.zeros(n)
+    lib().orc_restrict(dim, nx, ny, nz, _u(_b(cfixed)), int(tang), _d(_f64(rf)), _d(out))
+    return out
+
+
+def prolong_add(dim, fshape, ffixed, tang, ec, xf):
+    nx, ny, nz = fshape
+    xf = _f64(xf).copy()
+    lib().orc_prolong_add(dim, nx, ny, nz, _u(_b(ffixed)), int(tang), _d(_f64(ec)), _d(xf))
+    return xf
+
+
+def coarsen(dim, fshape, ffixed, fpen):
+    nx, ny, nz = fshape
+    cs = (nx // 2, ny // 2, nz // 2 if dim == 3 else 1)
+    n, _ = sizes(dim, *cs)
+    cf = np.zeros(n, np.uint8)
+    cp = np.zeros(n)
+    lib().orc_coarsen(dim, nx, ny, nz, _u(_b(ffixed)), _d(_f64(fpen)), _u(cf), _d(cp))
+    return cf, cp
+
+
+class MG:
+    """The multigrid preconditioner and FGMRES driver, restated on the CPU."""
+
+    def __init__(self, dim, shape, fixed, pen, nu_pre=2, nu_post=2, omega=0.8, tang=1, closed=0,
+                 coarsest=4):
+        nx, ny, nz = shape
+        self.dim, self.shape = dim, shape
+        self._fixed = _b(fixed)
+        self._pen = _f64(pen)
+        prm = np.array([nu_pre, nu_post, omega, tang, closed, coarsest, 0.0])
+        self.h = lib().orc_mg_create(dim, nx, ny, nz, _u(self._fixed), _d(self._pen), _d(prm))
+        self.n = self._pen.size
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_mg_free(self.h)
+            self.h = None
+
+    @property
+    def levels(self):
+        return lib().orc_mg_levels(self.h)
+
+    def level_dims(self, l):
+        d = (ctypes.c_int * 3)()
+        lib().orc_mg_level_dims(self.h, l, d)
+        return tuple(d)
+
+    def level_desc(self, l):
+        n, _ = sizes(self.dim, *self.level_dims(l))
+        f = np.zeros(n, np.uint8)
+        p = np.zeros(n)
+        lib().orc_mg_level_desc(self.h, l, _u(f), _d(p))
+        return f, p
+
+    def vcycle(self, b):
+        x = np.zeros(self.n)
+        lib().orc_mg_vcycle(self.h, _d(_f64(b)), _d(x))
+        return x
+
+    def apply(self, x, level=0):
+        y = np.zeros_like(_f64(x))
+        lib().orc_mg_apply(self.h, level, _d(_f64(x)), _d(y))
+        return y
+
+    def fgmres(self, b, x0=None, rtol=1e-10, restart=30, maxit=200):
+        x = np.zeros(self.n) if x0 is None else _f64(x0).copy()
+        hist = np.zeros(maxit + 2)
+        rr = ctypes.c_double(0)
+        it = lib().orc_fgmres(self.h, _d(_f64(b)), _d(x), float(rtol), int(restart), int(maxit),
+                              ctypes.byref(rr), _d(hist))
+        return x, it, rr.value, hist[: it + 1]

@@ -1,0 +1,65 @@
+// Launchers for the HIP kernels of st_kernels.hip (host-callable).
+#pragma once
+#include "st_common.h"
+
+namespace st {
+
+// alpha(r) = amax + (amin - amax) r(1+q)/(r+q) on r = rho^power, times scale
+// (FT/FluidTopoOpt3D.cpp:110-123)
+struct AlphaParams {
+    double amax, amin, q, power, scale;
+};
+
+template <typename TS> void launch_apply(const Lvl &, const uint8_t *, const TS *, const TS *, TS *, hipStream_t);
+template <typename TS>
+void launch_residual(const Lvl &, const uint8_t *, const TS *, const TS *b, const TS *x, TS *r, double *part,
+                     hipStream_t);
+unsigned residual_blocks(const Lvl &);
+template <typename TS>
+void launch_smooth(const Lvl &, const uint8_t *, const TS *, const TS *b, const TS *xin, TS *xout, int color,
+                   float omega, bool zero, hipStream_t);
+template <typename TS>
+void launch_restrict(const Lvl &gf, const Lvl &gc, const uint8_t *flc, const TS *rf, TS *bc, hipStream_t);
+template <typename TS>
+void launch_prolong(const Lvl &gf, const Lvl &gc, const uint8_t *flf, const TS *ec, TS *xf, hipStream_t);
+template <typename TF, typename TC>
+void launch_coarsen(const Lvl &gf, const Lvl &gc, const uint8_t *flf, const TF *alf, uint8_t *flc, TC *alc,
+                    hipStream_t);
+template <typename TS>
+void launch_probe(const Lvl &, const uint8_t *, const TS *, const long *dofs, int nd, int closed, double *A,
+                  hipStream_t);
+void launch_gauss_jordan(int n, double *A, int *perm, int *info, hipStream_t);
+template <typename TS>
+void launch_coarse_solve(int nd, const long *dofs, const double *Minv, const TS *b, TS *x, hipStream_t);
+template <typename TS>
+void launch_pack(const Lvl &, const double *abi, TS *pad, int fixed_only, const uint8_t *fl, double scale,
+                 hipStream_t);
+template <typename TS> void launch_unpack(const Lvl &, const TS *pad, double *abi, hipStream_t);
+void launch_flags(const Lvl &, const uint8_t *abi_fixed, uint8_t *fl, hipStream_t);
+template <typename TS> void launch_alpha_from_pen(const Lvl &, const double *pen, const uint8_t *fl, TS *al, hipStream_t);
+void launch_design_alpha(const Lvl &, const double *rho, const AlphaParams &, const uint8_t *fl, double *al,
+                         hipStream_t);
+void launch_sensitivity(const Lvl &, const double *rho, const AlphaParams &, const uint8_t *fl, const double *u,
+                        const double *lam, double *dj, hipStream_t);
+void launch_lift(const Lvl &, const uint8_t *fl, const double *xd, const double *body, double *b, hipStream_t);
+void launch_energy(const Lvl &, const uint8_t *fl, const double *al, const double *x, double *y, double *part,
+                   hipStream_t);
+unsigned energy_blocks(const Lvl &);
+void launch_filter(const Lvl &, double R, const double *W, const double *in, double *out, int transpose, hipStream_t);
+void launch_filter_w(const Lvl &, double R, double *W, hipStream_t);
+unsigned oc_blocks(long n);
+void launch_oc_step(long n, const double *rho, const double *dc, const double *dv, double lm, double move, double eta,
+                    double rho_min, double *rn, double *part, hipStream_t);
+unsigned absmax_blocks(long n);
+void launch_absdiff_max(long n, const double *a, const double *b, double *part, hipStream_t);
+void launch_sum(int np, int nvec, const double *part, double *out, hipStream_t);
+unsigned blas_blocks(long n);
+template <typename TA, typename TB> void launch_dot(long n, const TA *a, const TB *b, double *part, hipStream_t);
+template <typename TS>
+void launch_multidot(long n, const TS *w, const TS *V, long ld, int m, double *part, hipStream_t);
+template <typename TS>
+void launch_multiaxpy(long n, TS *w, const TS *V, long ld, int m, const double *h, double *part, hipStream_t);
+template <typename TX, typename TY> void launch_axpby(long n, double a, const TX *x, double b, TY *y, hipStream_t);
+template <typename TS> void launch_fill(long n, TS *y, double v, hipStream_t);
+
+}  // namespace st

@@ -1,0 +1,979 @@
+// Host orchestration of the HIP Stokes/Brinkman solver and the C ABI of
+// libstokes_hip.so (declared in include/stokes_to.h).
+//
+// Structure (MI355X-first, one device, one stream, all state resident in HBM):
+//  - level hierarchy n, n/2, ..., 4 in the padded node layout (st_common.h);
+//    the FP64 level data (flags, face alpha) is always kept, the multigrid
+//    preconditioner runs in FP64, FP32 or BF16 storage;
+//  - FGMRES(m) in FP64 on the device, right-preconditioned by one V-cycle, with
+//    classical Gram-Schmidt as two fused passes (multi-dot, multi-axpy + norm),
+//    stopped on the true FP64 residual.  Only the (m+1) Hessenberg scalars per
+//    iteration cross PCIe (the reference copies two full vectors per iteration
+//    and orthogonalises on the host, CX/GMRES.cpp:46-95).
+#include "../../include/stokes_to.h"
+#include "st_kernels.h"
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+using namespace st;
+
+namespace {
+
+template <typename TS> struct Hier {
+    std::vector<TS *> al, b, x, t, r;
+};
+
+struct Net;  // learned predictor (st_net.hip)
+
+}  // namespace
+
+struct st_ctx {
+    st_params prm;
+    int dim = 3, n = 0, L = 0;
+    std::vector<Lvl> lv;
+    std::vector<uint8_t *> fl;
+    Hier<double> h64;
+    Hier<float> h32;
+    Hier<bf16> hbf;
+    // coarsest dense inverse
+    long *d_dofs = nullptr;
+    int nd = 0;
+    double *d_minv = nullptr;
+    int *d_perm = nullptr, *d_info = nullptr;
+    // finest FP64 state (padded)
+    long ndof = 0, ncell = 0, nvec = 0;  // ABI sizes and padded vector length
+    double *X = nullptr, *B = nullptr, *XD = nullptr, *BODY = nullptr, *LAM = nullptr, *BA = nullptr, *ZERO = nullptr;
+    double *Wv = nullptr, *V = nullptr, *Zb = nullptr;
+    double *part = nullptr, *red = nullptr, *h_dev = nullptr;
+    int npart = 0;
+    // design
+    double *rho = nullptr, *rhof = nullptr, *dj = nullptr, *dc = nullptr, *rn = nullptr, *fW = nullptr, *dv = nullptr;
+    // staging
+    double *abi_a = nullptr, *abi_b = nullptr;
+    uint8_t *abi_u8 = nullptr;
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
+    bool geom = false, alpha = false, rhs = false, dirty = true, have_x = false;
+    std::string err;
+    std::vector<void *> allocs;
+    double last_setup_ms = 0.0;
+    void *net = nullptr;  // learned predictor state
+    // kernel timing: event pairs around finest-level launches
+    bool prof = false;
+    std::vector<hipEvent_t> evpool[ST_PROF_NKIND];
+    size_t evused[ST_PROF_NKIND] = {0, 0, 0};
+    double prof_ms[ST_PROF_NKIND] = {0, 0, 0};
+    long prof_n[ST_PROF_NKIND] = {0, 0, 0};
+};
+
+static void prof_mark(st_ctx *c, int kind)
+{
+    if (!c->prof) return;
+    auto &pool = c->evpool[kind];
+    if (c->evused[kind] >= pool.size()) {
+        hipEvent_t e;
+        hipEventCreate(&e);
+        pool.push_back(e);
+    }
+    hipEventRecord(pool[c->evused[kind]++], c->s);
+}
+
+// called after the stream is synchronised: fold recorded pairs into totals
+static void prof_collect(st_ctx *c)
+{
+    for (int k = 0; k < ST_PROF_NKIND; k++) {
+        for (size_t i = 0; i + 1 < c->evused[k]; i += 2) {
+            float ms = 0.f;
+            hipEventElapsedTime(&ms, c->evpool[k][i], c->evpool[k][i + 1]);
+            c->prof_ms[k] += ms;
+            c->prof_n[k] += 1;
+        }
+        c->evused[k] = 0;
+    }
+}
+
+#define CK(call)                                                                     \
+    do {                                                                             \
+        hipError_t e_ = (call);                                                      \
+        if (e_ != hipSuccess) {                                                      \
+            c->err = std::string(#call) + ": " + hipGetErrorString(e_);             \
+            return ST_EHIP;                                                          \
+        }                                                                            \
+    } while (0)
+#define CKR(expr)                 \
+    do {                          \
+        int r_ = (expr);          \
+        if (r_ != ST_OK) return r_; \
+    } while (0)
+
+static int fail(st_ctx *c, int code, const std::string &m)
+{
+    if (c) c->err = m;
+    return code;
+}
+
+template <typename T> static int dalloc(st_ctx *c, T **p, long n)
+{
+    void *q = nullptr;
+    hipError_t e = hipMalloc(&q, (size_t)(n > 0 ? n : 1) * sizeof(T));
+    if (e != hipSuccess) {
+        c->err = std::string("hipMalloc(") + std::to_string((long long)n * sizeof(T)) + " B): " + hipGetErrorString(e);
+        return ST_EHIP;
+    }
+    hipMemsetAsync(q, 0, (size_t)(n > 0 ? n : 1) * sizeof(T), c->s);
+    c->allocs.push_back(q);
+    *p = (T *)q;
+    return ST_OK;
+}
+
+static long abi_dofs(int dim, int n)
+{
+    if (dim == 3) return (long)n * n * n + 3L * (n + 1) * n * n;
+    return (long)n * n + 2L * (n + 1) * n;
+}
+
+// ---------------------------------------------------------------- helpers
+
+static int sum_partials(st_ctx *c, int np, int nvec, double *out_dev)
+{
+    launch_sum(np, nvec, c->part, out_dev, c->s);
+    return ST_OK;
+}
+
+static int dot64(st_ctx *c, const double *a, const double *b, long n, double *res)
+{
+    launch_dot<double, double>(n, a, b, c->part, c->s);
+    launch_sum((int)blas_blocks(n), 1, c->part, c->red, c->s);
+    CK(hipMemcpyAsync(res, c->red, sizeof(double), hipMemcpyDeviceToHost, c->s));
+    CK(hipStreamSynchronize(c->s));
+    return ST_OK;
+}
+
+// -------------------------------------------------------------- hierarchy
+
+template <typename TS> static Hier<TS> &hier(st_ctx *c);
+template <> Hier<double> &hier<double>(st_ctx *c) { return c->h64; }
+template <> Hier<float> &hier<float>(st_ctx *c) { return c->h32; }
+template <> Hier<bf16> &hier<bf16>(st_ctx *c) { return c->hbf; }
+
+template <typename TS> static int alloc_hier(st_ctx *c, bool level0_vectors)
+{
+    Hier<TS> &h = hier<TS>(c);
+    h.al.assign(c->L, nullptr);
+    h.b.assign(c->L, nullptr);
+    h.x.assign(c->L, nullptr);
+    h.t.assign(c->L, nullptr);
+    h.r.assign(c->L, nullptr);
+    for (int l = 0; l < c->L; l++) {
+        const Lvl &g = c->lv[l];
+        long nv = (long)g.nfield * g.S;
+        if (l > 0 || !std::is_same<TS, double>::value) CKR(dalloc(c, &h.al[l], (long)g.dim * g.S));
+        if (l > 0 || level0_vectors) {
+            CKR(dalloc(c, &h.b[l], nv));
+            CKR(dalloc(c, &h.x[l], nv));
+        }
+        CKR(dalloc(c, &h.t[l], nv));
+        CKR(dalloc(c, &h.r[l], nv));
+    }
+    return ST_OK;
+}
+
+template <typename TS> static void convert_alpha(st_ctx *c)
+{
+    Hier<TS> &h = hier<TS>(c);
+    for (int l = 0; l < c->L; l++) {
+        const Lvl &g = c->lv[l];
+        launch_axpby<double, TS>((long)g.dim * g.S, 1.0, c->h64.al[l], 0.0, h.al[l], c->s);
+    }
+}
+
+static int setup_hierarchy(st_ctx *c)
+{
+    if (!c->geom) return fail(c, ST_ESTATE, "geometry not set (st_set_geometry)");
+    if (!c->alpha) return fail(c, ST_ESTATE, "penalty/design not set (st_set_penalty or st_set_design)");
+    CK(hipEventRecord(c->e2, c->s));
+    for (int l = 0; l + 1 < c->L; l++)
+        launch_coarsen<double, double>(c->lv[l], c->lv[l + 1], c->fl[l], c->h64.al[l], c->fl[l + 1], c->h64.al[l + 1],
+                                       c->s);
+    if (c->prm.precision == ST_FP32) convert_alpha<float>(c);
+    if (c->prm.precision == ST_BF16) convert_alpha<bf16>(c);
+    const int Lc = c->L - 1;
+    launch_probe<double>(c->lv[Lc], c->fl[Lc], c->h64.al[Lc], c->d_dofs, c->nd, c->prm.closed, c->d_minv, c->s);
+    launch_gauss_jordan(c->nd, c->d_minv, c->d_perm, c->d_info, c->s);
+    int info = 0;
+    CK(hipMemcpyAsync(&info, c->d_info, sizeof(int), hipMemcpyDeviceToHost, c->s));
+    hipEvent_t e3;
+    CK(hipEventCreate(&e3));
+    CK(hipEventRecord(e3, c->s));
+    CK(hipStreamSynchronize(c->s));
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, c->e2, e3);
+    hipEventDestroy(e3);
+    c->last_setup_ms = ms;
+    if (info != 0)
+        return fail(c, ST_ESINGULAR,
+                    "coarsest-level matrix is singular at pivot " + std::to_string(info) +
+                        " (closed domain? set params.closed=1)");
+    c->dirty = false;
+    return ST_OK;
+}
+
+// ---------------------------------------------------------------- V-cycle
+
+template <typename TS> static void smooth(st_ctx *c, int l, const TS *b, TS *x, int nu, int first, bool zero)
+{
+    Hier<TS> &h = hier<TS>(c);
+    const Lvl &g = c->lv[l];
+    TS *t = h.t[l];
+    const float om = (float)c->prm.omega;
+    const bool pr = (l == 0);
+    for (int sw = 0; sw < nu; sw++) {
+        if (pr) prof_mark(c, ST_PROF_SMOOTH);
+        launch_smooth<TS>(g, c->fl[l], h.al[l], b, x, t, first, om, zero && sw == 0, c->s);
+        if (pr) { prof_mark(c, ST_PROF_SMOOTH); prof_mark(c, ST_PROF_SMOOTH); }
+        launch_smooth<TS>(g, c->fl[l], h.al[l], b, t, x, first ^ 1, om, false, c->s);
+        if (pr) prof_mark(c, ST_PROF_SMOOTH);
+    }
+}
+
+template <typename TS> static void vcycle(st_ctx *c, const TS *b0, TS *x0)
+{
+    Hier<TS> &h = hier<TS>(c);
+    const int L = c->L;
+    for (int l = 0; l + 1 < L; l++) {
+        const Lvl &g = c->lv[l];
+        const TS *b = (l == 0) ? b0 : h.b[l];
+        TS *x = (l == 0) ? x0 : h.x[l];
+        if (c->prm.nu_pre > 0) smooth<TS>(c, l, b, x, c->prm.nu_pre, 0, true);
+        else launch_fill<TS>((long)g.nfield * g.S, x, 0.0, c->s);
+        if (l == 0) prof_mark(c, ST_PROF_RESIDUAL);
+        launch_residual<TS>(g, c->fl[l], h.al[l], b, x, h.r[l], nullptr, c->s);
+        if (l == 0) prof_mark(c, ST_PROF_RESIDUAL);
+        launch_restrict<TS>(g, c->lv[l + 1], c->fl[l + 1], h.r[l], h.b[l + 1], c->s);
+    }
+    launch_coarse_solve<TS>(c->nd, c->d_dofs, c->d_minv, h.b[L - 1], h.x[L - 1], c->s);
+    for (int l = L - 2; l >= 0; l--) {
+        const TS *b = (l == 0) ? b0 : h.b[l];
+        TS *x = (l == 0) ? x0 : h.x[l];
+        launch_prolong<TS>(c->lv[l], c->lv[l + 1], c->fl[l], h.x[l + 1], x, c->s);
+        if (c->prm.nu_post > 0) smooth<TS>(c, l, b, x, c->prm.nu_post, 1, false);
+    }
+}
+
+// z = M v for FP64 Krylov vectors
+static void precondition(st_ctx *c, const double *v, double *z)
+{
+    const long nv = c->nvec;
+    if (c->prm.precision == ST_FP64) {
+        vcycle<double>(c, v, z);
+    } else if (c->prm.precision == ST_FP32) {
+        launch_axpby<double, float>(nv, 1.0, v, 0.0, c->h32.b[0], c->s);
+        vcycle<float>(c, c->h32.b[0], c->h32.x[0]);
+        launch_axpby<float, double>(nv, 1.0, c->h32.x[0], 0.0, z, c->s);
+    } else {
+        launch_axpby<double, bf16>(nv, 1.0, v, 0.0, c->hbf.b[0], c->s);
+        vcycle<bf16>(c, c->hbf.b[0], c->hbf.x[0]);
+        launch_axpby<bf16, double>(nv, 1.0, c->hbf.x[0], 0.0, z, c->s);
+    }
+}
+
+// ----------------------------------------------------------------- FGMRES
+
+// solves A x = b (padded FP64, device), x holds the initial guess on entry
+static int fgmres(st_ctx *c, const double *b, double *x, st_stats *st)
+{
+    const Lvl &g = c->lv[0];
+    const long nv = c->nvec;
+    const int m = c->prm.restart;
+    const double rtol = c->prm.rtol;
+    double bn2 = 0.0;
+    CKR(dot64(c, b, b, nv, &bn2));
+    double bn = std::sqrt(bn2);
+    if (!std::isfinite(bn)) return fail(c, ST_ENAN, "non-finite right-hand side");
+    if (bn == 0.0) bn = 1.0;
+    std::vector<double> H((size_t)(m + 1) * m, 0.0), cs(m), sn(m), gv(m + 1), y(m), hh(m + 2);
+    int it = 0;
+    double rr = 0.0, rr0 = -1.0;
+    const int rb = (int)residual_blocks(g);
+    for (;;) {
+        double *v0 = c->V;
+        launch_residual<double>(g, c->fl[0], c->h64.al[0], b, x, v0, c->part, c->s);
+        launch_sum(rb, 1, c->part, c->red, c->s);
+        double r2 = 0.0;
+        CK(hipMemcpyAsync(&r2, c->red, sizeof(double), hipMemcpyDeviceToHost, c->s));
+        CK(hipStreamSynchronize(c->s));
+        double beta = std::sqrt(r2);
+        if (!std::isfinite(beta)) return fail(c, ST_ENAN, "non-finite residual");
+        rr = beta / bn;
+        if (rr0 < 0) rr0 = rr;
+        if (rr <= rtol || it >= c->prm.max_iter) break;
+        launch_axpby<double, double>(nv, 1.0 / beta, v0, 0.0, v0, c->s);
+        std::fill(gv.begin(), gv.end(), 0.0);
+        gv[0] = beta;
+        int j = 0;
+        for (; j < m && it < c->prm.max_iter; j++) {
+            double *vj = c->V + (long)j * nv, *zj = c->Zb + (long)j * nv;
+            precondition(c, vj, zj);
+            prof_mark(c, ST_PROF_APPLY);
+            launch_apply<double>(g, c->fl[0], c->h64.al[0], zj, c->Wv, c->s);
+            prof_mark(c, ST_PROF_APPLY);
+            const unsigned nbk = blas_blocks(nv);
+            launch_multidot<double>(nv, c->Wv, c->V, nv, j + 1, c->part, c->s);
+            launch_sum((int)nbk, j + 1, c->part, c->h_dev, c->s);
+            launch_multiaxpy<double>(nv, c->Wv, c->V, nv, j + 1, c->h_dev, c->part, c->s);
+            launch_sum((int)nbk, 1, c->part, c->h_dev + (j + 1), c->s);
+            CK(hipMemcpyAsync(hh.data(), c->h_dev, sizeof(double) * (j + 2), hipMemcpyDeviceToHost, c->s));
+            CK(hipStreamSynchronize(c->s));
+            const double hn = std::sqrt(hh[j + 1]);
+            if (!std::isfinite(hn)) return fail(c, ST_ENAN, "non-finite Arnoldi vector");
+            for (int i = 0; i <= j; i++) H[(size_t)i * m + j] = hh[i];
+            H[(size_t)(j + 1) * m + j] = hn;
+            if (hn > 0) launch_axpby<double, double>(nv, 1.0 / hn, c->Wv, 0.0, c->V + (long)(j + 1) * nv, c->s);
+            for (int i = 0; i < j; i++) {
+                double a = H[(size_t)i * m + j], bb = H[(size_t)(i + 1) * m + j];
+                H[(size_t)i * m + j] = cs[i] * a + sn[i] * bb;
+                H[(size_t)(i + 1) * m + j] = -sn[i] * a + cs[i] * bb;
+            }
+            double a = H[(size_t)j * m + j], bb = H[(size_t)(j + 1) * m + j];
+            double den = std::sqrt(a * a + bb * bb);
+            cs[j] = a / den;
+            sn[j] = bb / den;
+            H[(size_t)j * m + j] = den;
+            H[(size_t)(j + 1) * m + j] = 0.0;
+            gv[j + 1] = -sn[j] * gv[j];
+            gv[j] = cs[j] * gv[j];
+            it++;
+            if (std::fabs(gv[j + 1]) / bn <= rtol || hn == 0.0) {
+                j++;
+                break;
+            }
+        }
+        for (int i = j - 1; i >= 0; i--) {
+            double sacc = gv[i];
+            for (int k = i + 1; k < j; k++) sacc -= H[(size_t)i * m + k] * y[k];
+            y[i] = sacc / H[(size_t)i * m + i];
+        }
+        for (int i = 0; i < j; i++) y[i] = -y[i];
+        CK(hipMemcpyAsync(c->h_dev, y.data(), sizeof(double) * j, hipMemcpyHostToDevice, c->s));
+        launch_multiaxpy<double>(nv, x, c->Zb, nv, j, c->h_dev, nullptr, c->s);
+    }
+    if (st) {
+        st->iters = it;
+        st->relres = rr;
+        st->relres0 = rr0;
+        st->converged = rr <= rtol;
+    }
+    return rr <= rtol ? ST_OK : ST_ENOCONV;
+}
+
+// ------------------------------------------------------------- ABI core
+
+static int stage_in(st_ctx *c, const double *h, long n)
+{
+    CK(hipMemcpyAsync(c->abi_a, h, sizeof(double) * n, hipMemcpyHostToDevice, c->s));
+    return ST_OK;
+}
+static int stage_out(st_ctx *c, double *h, long n)
+{
+    CK(hipMemcpyAsync(h, c->abi_b, sizeof(double) * n, hipMemcpyDeviceToHost, c->s));
+    CK(hipStreamSynchronize(c->s));
+    return ST_OK;
+}
+
+static int check_finite(st_ctx *c, const double *h, long n, const char *what)
+{
+    for (long q = 0; q < n; q++)
+        if (!std::isfinite(h[q])) return fail(c, ST_ENAN, std::string("non-finite value in ") + what);
+    return ST_OK;
+}
+
+extern "C" {
+
+void st_default_params(st_params *p, int dim, int n)
+{
+    std::memset(p, 0, sizeof(*p));
+    p->dim = dim;
+    p->n = n;
+    p->alpha_max = 1e4;
+    p->alpha_min = 0.0;
+    p->q = 0.1;
+    p->power = 1.0;
+    p->alpha_scale = (n > 0) ? 1.0 / ((double)n * n) : 1.0;
+    p->precision = ST_FP64;
+    p->rtol = 1e-8;
+    p->max_iter = 200;
+    p->restart = 30;
+    p->nu_pre = 2;
+    p->nu_post = 2;
+    p->omega = 1.0;
+    p->closed = 0;
+    p->filter_radius = 1.5;
+}
+
+int st_create(st_ctx **out, const st_params *p)
+{
+    if (!out || !p) return ST_EINVAL;
+    *out = nullptr;
+    if (p->dim != 2 && p->dim != 3) return ST_EINVAL;
+    int n = p->n;
+    if (n < 8 || (n % 4) != 0) return ST_EINVAL;
+    {
+        int t = n / 4;
+        while (t % 2 == 0) t /= 2;
+        if (t != 1) return ST_EINVAL;  // n = 4 * 2^k
+    }
+    if (p->restart < 1 || p->restart > 32 || p->max_iter < 1 || !(p->rtol > 0)) return ST_EINVAL;
+    if (p->precision < ST_FP64 || p->precision > ST_BF16) return ST_EINVAL;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return ST_EHIP;
+    st_ctx *c = new st_ctx();
+    c->prm = *p;
+    c->dim = p->dim;
+    c->n = n;
+    if (hipStreamCreate(&c->s) != hipSuccess) {
+        delete c;
+        return ST_EHIP;
+    }
+    hipEventCreate(&c->e0);
+    hipEventCreate(&c->e1);
+    hipEventCreate(&c->e2);
+    int m = n;
+    for (;;) {
+        c->lv.push_back(make_lvl(c->dim, m, m, m));
+        if (m % 2 != 0 || m / 2 < 4) break;
+        m /= 2;
+    }
+    c->L = (int)c->lv.size();
+    int rc = ST_OK;
+    c->fl.assign(c->L, nullptr);
+    for (int l = 0; l < c->L && rc == ST_OK; l++) rc = dalloc(c, &c->fl[l], c->lv[l].S);
+    const Lvl &g0 = c->lv[0];
+    c->nvec = (long)g0.nfield * g0.S;
+    c->ndof = abi_dofs(c->dim, n);
+    c->ncell = (c->dim == 3) ? (long)n * n * n : (long)n * n;
+    const int mr = p->restart;
+    if (rc == ST_OK) rc = dalloc(c, &c->h64.al.emplace_back(), (long)c->dim * g0.S);
+    if (rc == ST_OK) {
+        double *al0 = c->h64.al[0];
+        c->h64.al.clear();
+        rc = alloc_hier<double>(c, false);
+        if (rc == ST_OK) c->h64.al[0] = al0;
+    }
+    if (rc == ST_OK && p->precision == ST_FP32) rc = alloc_hier<float>(c, true);
+    if (rc == ST_OK && p->precision == ST_BF16) rc = alloc_hier<bf16>(c, true);
+    if (rc == ST_OK) rc = dalloc(c, &c->X, c->nvec);
+    if (rc == ST_OK) rc = dalloc(c, &c->B, c->nvec);
+    if (rc == ST_OK) rc = dalloc(c, &c->XD, c->nvec);
+    if (rc == ST_OK) rc = dalloc(c, &c->BODY, c->nvec);
+    if (rc == ST_OK) rc = dalloc(c, &c->LAM, c->nvec);
+    if (rc == ST_OK) rc = dalloc(c, &c->BA, c->nvec);
+    if (rc == ST_OK) rc = dalloc(c, &c->ZERO, c->nvec);
+    if (rc == ST_OK) rc = dalloc(c, &c->Wv, c->nvec);
+    if (rc == ST_OK) rc = dalloc(c, &c->V, c->nvec * (mr + 1));
+    if (rc == ST_OK) rc = dalloc(c, &c->Zb, c->nvec * mr);
+    c->npart = 8192 * 33;
+    if (rc == ST_OK) rc = dalloc(c, &c->part, c->npart);
+    if (rc == ST_OK) rc = dalloc(c, &c->red, 64);
+    if (rc == ST_OK) rc = dalloc(c, &c->h_dev, 64);
+    for (double **pp : {&c->rho, &c->rhof, &c->dj, &c->dc, &c->rn, &c->fW, &c->dv})
+        if (rc == ST_OK) rc = dalloc(c, pp, c->ncell);
+    if (rc == ST_OK) rc = dalloc(c, &c->abi_a, c->ndof);
+    if (rc == ST_OK) rc = dalloc(c, &c->abi_b, c->ndof);
+    if (rc == ST_OK) rc = dalloc(c, &c->abi_u8, c->ndof);
+    // coarsest compact DOF list (all valid DOFs of the padded coarsest level)
+    if (rc == ST_OK) {
+        const Lvl &gc = c->lv[c->L - 1];
+        std::vector<long> dofs;
+        for (int f = 0; f < gc.nfield; f++)
+            for (int i = 0; i < gc.Px; i++)
+                for (int j = 0; j < gc.Py; j++)
+                    for (int k = 0; k < gc.Pz; k++) {
+                        int A = gc.nx + (f == 1), Bv = gc.ny + (f == 2), Cv = gc.nz + (f == 3);
+                        if (i < A && j < Bv && k < Cv) dofs.push_back((long)f * gc.S + ((long)i * gc.Py + j) * gc.Pz + k);
+                    }
+        c->nd = (int)dofs.size();
+        rc = dalloc(c, &c->d_dofs, c->nd);
+        if (rc == ST_OK) rc = dalloc(c, &c->d_minv, (long)c->nd * c->nd);
+        if (rc == ST_OK) rc = dalloc(c, &c->d_perm, c->nd);
+        if (rc == ST_OK) rc = dalloc(c, &c->d_info, 4);
+        if (rc == ST_OK && hipMemcpy(c->d_dofs, dofs.data(), sizeof(long) * c->nd, hipMemcpyHostToDevice) != hipSuccess)
+            rc = ST_EHIP;
+    }
+    if (rc == ST_OK) {
+        launch_filter_w(c->lv[0], p->filter_radius > 1.0 ? p->filter_radius : 1.0, c->fW, c->s);
+        if (hipStreamSynchronize(c->s) != hipSuccess) rc = ST_EHIP;
+    }
+    if (rc != ST_OK) {
+        std::fprintf(stderr, "st_create: %s\n", c->err.c_str());
+        st_destroy(c);
+        return rc;
+    }
+    *out = c;
+    return ST_OK;
+}
+
+void st_destroy(st_ctx *c)
+{
+    if (!c) return;
+    if (c->s) hipStreamSynchronize(c->s);
+    for (void *q : c->allocs) hipFree(q);
+    if (c->e0) hipEventDestroy(c->e0);
+    if (c->e1) hipEventDestroy(c->e1);
+    if (c->e2) hipEventDestroy(c->e2);
+    if (c->s) hipStreamDestroy(c->s);
+    delete c;
+}
+
+const char *st_last_error(const st_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int st_get_params(const st_ctx *c, st_params *p)
+{
+    if (!c || !p) return ST_EINVAL;
+    *p = c->prm;
+    return ST_OK;
+}
+long st_num_dofs(const st_ctx *c) { return c ? c->ndof : -1; }
+long st_num_cells(const st_ctx *c) { return c ? c->ncell : -1; }
+
+int st_set_geometry(st_ctx *c, const uint8_t *fixed)
+{
+    if (!c || !fixed) return ST_EINVAL;
+    CK(hipMemcpyAsync(c->abi_u8, fixed, c->ndof, hipMemcpyHostToDevice, c->s));
+    launch_flags(c->lv[0], c->abi_u8, c->fl[0], c->s);
+    CK(hipStreamSynchronize(c->s));
+    c->geom = true;
+    c->dirty = true;
+    return ST_OK;
+}
+
+int st_set_penalty(st_ctx *c, const double *pen)
+{
+    if (!c || !pen) return ST_EINVAL;
+    if (!c->geom) return fail(c, ST_ESTATE, "st_set_geometry must precede st_set_penalty");
+    CKR(check_finite(c, pen, c->ndof, "penalty"));
+    for (long q = 0; q < c->ncell; q++)
+        if (pen[q] != 0.0) return fail(c, ST_EINVAL, "nonzero cell penalty is not supported");
+    CKR(stage_in(c, pen, c->ndof));
+    launch_alpha_from_pen<double>(c->lv[0], c->abi_a, c->fl[0], c->h64.al[0], c->s);
+    CK(hipStreamSynchronize(c->s));
+    c->alpha = true;
+    c->dirty = true;
+    return ST_OK;
+}
+
+static AlphaParams alpha_params(const st_ctx *c)
+{
+    AlphaParams a;
+    a.amax = c->prm.alpha_max;
+    a.amin = c->prm.alpha_min;
+    a.q = c->prm.q;
+    a.power = c->prm.power;
+    a.scale = c->prm.alpha_scale;
+    return a;
+}
+
+static void design_to_alpha(st_ctx *c, const double *rho_dev)
+{
+    launch_design_alpha(c->lv[0], rho_dev, alpha_params(c), c->fl[0], c->h64.al[0], c->s);
+    c->alpha = true;
+    c->dirty = true;
+}
+
+int st_set_design(st_ctx *c, const double *rho)
+{
+    if (!c || !rho) return ST_EINVAL;
+    if (!c->geom) return fail(c, ST_ESTATE, "st_set_geometry must precede st_set_design");
+    CKR(check_finite(c, rho, c->ncell, "design"));
+    CK(hipMemcpyAsync(c->rho, rho, sizeof(double) * c->ncell, hipMemcpyHostToDevice, c->s));
+    design_to_alpha(c, c->rho);
+    CK(hipStreamSynchronize(c->s));
+    return ST_OK;
+}
+
+int st_set_design_device(st_ctx *c, const double *d_rho)
+{
+    if (!c || !d_rho) return ST_EINVAL;
+    if (!c->geom) return fail(c, ST_ESTATE, "st_set_geometry must precede st_set_design_device");
+    CK(hipMemcpyAsync(c->rho, d_rho, sizeof(double) * c->ncell, hipMemcpyDeviceToDevice, c->s));
+    design_to_alpha(c, c->rho);
+    CK(hipStreamSynchronize(c->s));
+    return ST_OK;
+}
+
+int st_profile(st_ctx *c, int on)
+{
+    if (!c) return ST_EINVAL;
+    CK(hipStreamSynchronize(c->s));
+    prof_collect(c);
+    if (on) {
+        for (int k = 0; k < ST_PROF_NKIND; k++) { c->prof_ms[k] = 0.0; c->prof_n[k] = 0; }
+    }
+    c->prof = on != 0;
+    return ST_OK;
+}
+
+int st_profile_read(st_ctx *c, int kind, double *total_ms, long *launches, double *bytes)
+{
+    if (!c || kind < 0 || kind >= ST_PROF_NKIND) return ST_EINVAL;
+    if (total_ms) *total_ms = c->prof_ms[kind];
+    if (launches) *launches = c->prof_n[kind];
+    if (bytes) {
+        // algorithmic HBM bytes of one finest-level launch over the valid DOFs, with the
+        // masks counted as bit fields (n_f/8 bytes):
+        //   operator apply      s(2 n_c + 3 n_f) + n_f/8   (x read, alpha read, y write)
+        //   residual            s(3 n_c + 4 n_f) + n_f/8   (x, b, alpha read, r write)
+        //   smoother colour     half of a two-colour sweep s(3 n_c + 4 n_f) + n_f/8
+        //                       (x, b, alpha read once, x written once per sweep)
+        const double nc = (double)c->ncell, nf = (double)(c->ndof - c->ncell);
+        double sv = 8.0;
+        if (kind != ST_PROF_APPLY) sv = (c->prm.precision == ST_FP32) ? 4.0 : (c->prm.precision == ST_BF16 ? 2.0 : 8.0);
+        if (kind == ST_PROF_SMOOTH) *bytes = 0.5 * (sv * (3.0 * nc + 4.0 * nf) + nf / 8.0);
+        else if (kind == ST_PROF_APPLY) *bytes = sv * (2.0 * nc + 3.0 * nf) + nf / 8.0;
+        else *bytes = sv * (3.0 * nc + 4.0 * nf) + nf / 8.0;
+    }
+    return ST_OK;
+}
+
+int st_get_design(st_ctx *c, double *rho)
+{
+    if (!c || !rho) return ST_EINVAL;
+    CK(hipMemcpyAsync(rho, c->rho, sizeof(double) * c->ncell, hipMemcpyDeviceToHost, c->s));
+    CK(hipStreamSynchronize(c->s));
+    return ST_OK;
+}
+
+int st_set_rhs(st_ctx *c, const double *body, const double *dirichlet)
+{
+    if (!c) return ST_EINVAL;
+    if (!c->geom) return fail(c, ST_ESTATE, "st_set_geometry must precede st_set_rhs");
+    const Lvl &g = c->lv[0];
+    if (dirichlet) {
+        CKR(check_finite(c, dirichlet, c->ndof, "dirichlet"));
+        CKR(stage_in(c, dirichlet, c->ndof));
+        launch_pack<double>(g, c->abi_a, c->XD, 1, c->fl[0], 1.0, c->s);
+    } else {
+        launch_fill<double>(c->nvec, c->XD, 0.0, c->s);
+    }
+    if (body) {
+        CKR(check_finite(c, body, c->ndof, "body"));
+        CKR(stage_in(c, body, c->ndof));
+        launch_pack<double>(g, c->abi_a, c->BODY, 0, c->fl[0], 1.0, c->s);
+    } else {
+        launch_fill<double>(c->nvec, c->BODY, 0.0, c->s);
+    }
+    launch_lift(g, c->fl[0], c->XD, c->BODY, c->B, c->s);
+    CK(hipStreamSynchronize(c->s));
+    c->rhs = true;
+    return ST_OK;
+}
+
+static int solve_core(st_ctx *c, const double *b, double *x, const double *xd, int init, st_stats *st)
+{
+    if (!c->geom || !c->alpha) return fail(c, ST_ESTATE, "geometry and penalty/design must be set before solving");
+    st_stats local;
+    std::memset(&local, 0, sizeof(local));
+    CK(hipEventRecord(c->e0, c->s));
+    if (c->dirty) CKR(setup_hierarchy(c));
+    local.setup_ms = c->last_setup_ms;
+    if (init == ST_INIT_ZERO) launch_axpby<double, double>(c->nvec, 1.0, xd, 0.0, x, c->s);
+    int rc = fgmres(c, b, x, &local);
+    CK(hipEventRecord(c->e1, c->s));
+    CK(hipEventSynchronize(c->e1));
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, c->e0, c->e1);
+    local.ms = ms;
+    prof_collect(c);
+    if (st) *st = local;
+    return rc;
+}
+
+int st_solve(st_ctx *c, int init, const double *x0, double *x_out, st_stats *stats)
+{
+    if (!c) return ST_EINVAL;
+    if (!c->rhs) return fail(c, ST_ESTATE, "st_set_rhs must precede st_solve");
+    if (init == ST_INIT_HOST) {
+        if (!x0) return ST_EINVAL;
+        CKR(check_finite(c, x0, c->ndof, "x0"));
+        CKR(stage_in(c, x0, c->ndof));
+        launch_pack<double>(c->lv[0], c->abi_a, c->X, 0, c->fl[0], 1.0, c->s);
+    } else if (init == ST_INIT_KEEP) {
+        if (!c->have_x) init = ST_INIT_ZERO;
+    } else if (init == ST_INIT_LEARNED) {
+        return fail(c, ST_EINVAL, "learned initial guess is not available in this build");
+    } else if (init != ST_INIT_ZERO) {
+        return ST_EINVAL;
+    }
+    int rc = solve_core(c, c->B, c->X, c->XD, init == ST_INIT_HOST ? ST_INIT_KEEP : init, stats);
+    if (rc == ST_OK || rc == ST_ENOCONV) c->have_x = true;
+    if (x_out && (rc == ST_OK || rc == ST_ENOCONV)) {
+        launch_unpack<double>(c->lv[0], c->X, c->abi_b, c->s);
+        int r2 = stage_out(c, x_out, c->ndof);
+        if (r2 != ST_OK) return r2;
+    }
+    return rc;
+}
+
+int st_get_solution(st_ctx *c, double *x)
+{
+    if (!c || !x) return ST_EINVAL;
+    launch_unpack<double>(c->lv[0], c->X, c->abi_b, c->s);
+    return stage_out(c, x, c->ndof);
+}
+
+int st_apply(st_ctx *c, const double *x, double *y)
+{
+    if (!c || !x || !y) return ST_EINVAL;
+    if (!c->geom || !c->alpha) return fail(c, ST_ESTATE, "geometry and penalty/design must be set");
+    CKR(stage_in(c, x, c->ndof));
+    launch_pack<double>(c->lv[0], c->abi_a, c->Wv, 0, c->fl[0], 1.0, c->s);
+    launch_apply<double>(c->lv[0], c->fl[0], c->h64.al[0], c->Wv, c->LAM, c->s);
+    launch_unpack<double>(c->lv[0], c->LAM, c->abi_b, c->s);
+    return stage_out(c, y, c->ndof);
+}
+
+int st_apply_device(st_ctx *c, const double *d_x, double *d_y)
+{
+    if (!c || !d_x || !d_y) return ST_EINVAL;
+    if (!c->geom || !c->alpha) return fail(c, ST_ESTATE, "geometry and penalty/design must be set");
+    launch_pack<double>(c->lv[0], d_x, c->Wv, 0, c->fl[0], 1.0, c->s);
+    launch_apply<double>(c->lv[0], c->fl[0], c->h64.al[0], c->Wv, c->LAM, c->s);
+    launch_unpack<double>(c->lv[0], c->LAM, d_y, c->s);
+    CK(hipStreamSynchronize(c->s));
+    return ST_OK;
+}
+
+static int objective_dev(st_ctx *c, const double *x, double *y, double *J)
+{
+    const Lvl &g = c->lv[0];
+    launch_energy(g, c->fl[0], c->h64.al[0], x, y, c->part, c->s);
+    launch_sum((int)energy_blocks(g), 1, c->part, c->red, c->s);
+    double s = 0.0;
+    CK(hipMemcpyAsync(&s, c->red, sizeof(double), hipMemcpyDeviceToHost, c->s));
+    CK(hipStreamSynchronize(c->s));
+    if (J) *J = 0.5 * s;
+    return ST_OK;
+}
+
+int st_objective(st_ctx *c, const double *x, double *J)
+{
+    if (!c || !x || !J) return ST_EINVAL;
+    if (!c->geom || !c->alpha) return fail(c, ST_ESTATE, "geometry and penalty/design must be set");
+    CKR(stage_in(c, x, c->ndof));
+    launch_pack<double>(c->lv[0], c->abi_a, c->Wv, 0, c->fl[0], 1.0, c->s);
+    return objective_dev(c, c->Wv, nullptr, J);
+}
+
+int st_energy_grad(st_ctx *c, const double *x, double *gx)
+{
+    if (!c || !x || !gx) return ST_EINVAL;
+    if (!c->geom || !c->alpha) return fail(c, ST_ESTATE, "geometry and penalty/design must be set");
+    CKR(stage_in(c, x, c->ndof));
+    launch_pack<double>(c->lv[0], c->abi_a, c->Wv, 0, c->fl[0], 1.0, c->s);
+    CKR(objective_dev(c, c->Wv, c->BA, nullptr));
+    launch_unpack<double>(c->lv[0], c->BA, c->abi_b, c->s);
+    return stage_out(c, gx, c->ndof);
+}
+
+int st_sensitivity(st_ctx *c, const double *rho, const double *x, const double *lam, double *dj)
+{
+    if (!c || !rho || !x || !lam || !dj) return ST_EINVAL;
+    if (!c->geom) return fail(c, ST_ESTATE, "geometry must be set");
+    CK(hipMemcpyAsync(c->rn, rho, sizeof(double) * c->ncell, hipMemcpyHostToDevice, c->s));
+    CKR(stage_in(c, x, c->ndof));
+    launch_pack<double>(c->lv[0], c->abi_a, c->Wv, 0, c->fl[0], 1.0, c->s);
+    CKR(stage_in(c, lam, c->ndof));
+    launch_pack<double>(c->lv[0], c->abi_a, c->BA, 0, c->fl[0], 1.0, c->s);
+    launch_sensitivity(c->lv[0], c->rn, alpha_params(c), c->fl[0], c->Wv, c->BA, c->dj, c->s);
+    CK(hipMemcpyAsync(dj, c->dj, sizeof(double) * c->ncell, hipMemcpyDeviceToHost, c->s));
+    CK(hipStreamSynchronize(c->s));
+    return ST_OK;
+}
+
+static void filter_dev(st_ctx *c, const double *in, double *out, int transpose)
+{
+    if (c->prm.filter_radius > 1.0)
+        launch_filter(c->lv[0], c->prm.filter_radius, c->fW, in, out, transpose, c->s);
+    else
+        launch_axpby<double, double>(c->ncell, 1.0, in, 0.0, out, c->s);
+}
+
+int st_filter(st_ctx *c, const double *in, double *out, int transpose)
+{
+    if (!c || !in || !out) return ST_EINVAL;
+    CK(hipMemcpyAsync(c->rn, in, sizeof(double) * c->ncell, hipMemcpyHostToDevice, c->s));
+    filter_dev(c, c->rn, c->dc, transpose);
+    CK(hipMemcpyAsync(out, c->dc, sizeof(double) * c->ncell, hipMemcpyDeviceToHost, c->s));
+    CK(hipStreamSynchronize(c->s));
+    return ST_OK;
+}
+
+// OC with bisection on the device: rho (dev), dc (dev) -> rn (dev)
+static int oc_dev(st_ctx *c, const double *rho, const double *dc, double volfrac, double move, double eta,
+                  double rho_min, double *rn, double *change, double *vol_out)
+{
+    const long n = c->ncell;
+    launch_fill<double>(n, c->rhof, 1.0 / (double)n, c->s);
+    filter_dev(c, c->rhof, c->dv, 1);  // dV/drho = H^T(1/(n W))
+    const int nb = (int)oc_blocks(n);
+    double l1 = 0.0, l2 = 1e9, vol = 0.0;
+    int it = 0;
+    while ((l2 - l1) > 1e-13 * (l1 + l2) && it < 200) {
+        double lm = 0.5 * (l1 + l2);
+        launch_oc_step(n, rho, dc, c->dv, lm, move, eta, rho_min, rn, c->part, c->s);
+        launch_sum(nb, 1, c->part, c->red, c->s);
+        CK(hipMemcpyAsync(&vol, c->red, sizeof(double), hipMemcpyDeviceToHost, c->s));
+        CK(hipStreamSynchronize(c->s));
+        if (vol > volfrac) l1 = lm;
+        else l2 = lm;
+        it++;
+    }
+    launch_absdiff_max(n, rn, rho, c->part, c->s);
+    std::vector<double> pm(absmax_blocks(n));
+    CK(hipMemcpyAsync(pm.data(), c->part, sizeof(double) * pm.size(), hipMemcpyDeviceToHost, c->s));
+    CK(hipStreamSynchronize(c->s));
+    double ch = 0.0;
+    for (double v : pm) ch = v > ch ? v : ch;
+    if (change) *change = ch;
+    if (vol_out) *vol_out = vol;
+    return ST_OK;
+}
+
+int st_oc_update(st_ctx *c, const double *rho, const double *dcp, double volfrac, double move, double eta,
+                 double rho_min, double *rho_new, double *change)
+{
+    if (!c || !rho || !dcp || !rho_new) return ST_EINVAL;
+    CK(hipMemcpyAsync(c->rn, rho, sizeof(double) * c->ncell, hipMemcpyHostToDevice, c->s));
+    CK(hipMemcpyAsync(c->dc, dcp, sizeof(double) * c->ncell, hipMemcpyHostToDevice, c->s));
+    CKR(oc_dev(c, c->rn, c->dc, volfrac, move, eta, rho_min, c->dj, change, nullptr));
+    CK(hipMemcpyAsync(rho_new, c->dj, sizeof(double) * c->ncell, hipMemcpyDeviceToHost, c->s));
+    CK(hipStreamSynchronize(c->s));
+    return ST_OK;
+}
+
+int st_to_step(st_ctx *c, double volfrac, double move, double eta, double rho_min, double *J, double *vol,
+               double *change, st_stats *fwd, st_stats *adj)
+{
+    if (!c) return ST_EINVAL;
+    if (!c->geom || !c->rhs) return fail(c, ST_ESTATE, "geometry and rhs must be set before st_to_step");
+    // filtered physical density -> face alpha
+    filter_dev(c, c->rho, c->rhof, 0);
+    design_to_alpha(c, c->rhof);
+    int rc = solve_core(c, c->B, c->X, c->XD, ST_INIT_ZERO, fwd);
+    if (rc != ST_OK) return rc;
+    c->have_x = true;
+    CKR(objective_dev(c, c->X, c->BA, J));
+    // adjoint: A lam = G x (A symmetric), homogeneous Dirichlet data
+    rc = solve_core(c, c->BA, c->LAM, c->ZERO, ST_INIT_ZERO, adj);
+    if (rc != ST_OK) return rc;
+    launch_sensitivity(c->lv[0], c->rhof, alpha_params(c), c->fl[0], c->X, c->LAM, c->dj, c->s);
+    filter_dev(c, c->dj, c->dc, 1);
+    CKR(oc_dev(c, c->rho, c->dc, volfrac, move, eta, rho_min, c->rn, change, vol));
+    launch_axpby<double, double>(c->ncell, 1.0, c->rn, 0.0, c->rho, c->s);
+    CK(hipStreamSynchronize(c->s));
+    return ST_OK;
+}
+
+int st_record_sample(st_ctx *c) { return c ? fail(c, ST_EINVAL, "learned predictor not built") : ST_EINVAL; }
+int st_net_train(st_ctx *c, int, double, double *) { return c ? fail(c, ST_EINVAL, "learned predictor not built") : ST_EINVAL; }
+
+// ------------------------------------------------------ MG test surface
+
+int st_mg_levels(const st_ctx *c) { return c ? c->L : -1; }
+int st_mg_level_n(const st_ctx *c, int l) { return (c && l >= 0 && l < c->L) ? c->lv[l].nx : -1; }
+
+int st_mg_vcycle(st_ctx *c, const double *b, double *z)
+{
+    if (!c || !b || !z) return ST_EINVAL;
+    if (c->dirty) CKR(setup_hierarchy(c));
+    CKR(stage_in(c, b, c->ndof));
+    launch_pack<double>(c->lv[0], c->abi_a, c->BA, 0, c->fl[0], 1.0, c->s);
+    precondition(c, c->BA, c->Wv);
+    launch_unpack<double>(c->lv[0], c->Wv, c->abi_b, c->s);
+    return stage_out(c, z, c->ndof);
+}
+
+static long level_dofs(const st_ctx *c, int l) { return abi_dofs(c->dim, c->lv[l].nx); }
+
+int st_mg_level_desc(st_ctx *c, int l, uint8_t *fixed, double *alpha_faces)
+{
+    if (!c || l < 0 || l >= c->L) return ST_EINVAL;
+    if (c->dirty) CKR(setup_hierarchy(c));
+    const Lvl &g = c->lv[l];
+    // flags -> ABI fixed flags via a float round trip of a mask vector
+    std::vector<uint8_t> hfl(g.S);
+    CK(hipMemcpyAsync(hfl.data(), c->fl[l], g.S, hipMemcpyDeviceToHost, c->s));
+    CK(hipStreamSynchronize(c->s));
+    long nd = level_dofs(c, l);
+    long o = 0;
+    for (int f = 0; f < g.nfield; f++) {
+        int A = g.nx + (f == 1), Bv = g.ny + (f == 2), Cv = g.nz + (f == 3);
+        for (int i = 0; i < A; i++)
+            for (int j = 0; j < Bv; j++)
+                for (int k = 0; k < Cv; k++) {
+                    long nn = ((long)i * g.Py + j) * g.Pz + k;
+                    if (fixed) fixed[o] = ((hfl[nn] >> f) & 1) ? 0 : 1;
+                    o++;
+                }
+    }
+    if (o != nd) return fail(c, ST_EINVAL, "internal: level dof count mismatch");
+    if (alpha_faces) {
+        // pack alpha faces as the face part of an ABI vector (cells zero)
+        launch_fill<double>((long)g.nfield * g.S, c->Wv, 0.0, c->s);
+        CK(hipMemcpyAsync(c->Wv + g.S, c->h64.al[l], sizeof(double) * g.dim * g.S, hipMemcpyDeviceToDevice, c->s));
+        launch_unpack<double>(g, c->Wv, c->abi_b, c->s);
+        std::vector<double> tmp(nd);
+        CKR(stage_out(c, tmp.data(), nd));
+        long nc = (long)g.nx * g.ny * g.nz;
+        std::memcpy(alpha_faces, tmp.data() + nc, sizeof(double) * (nd - nc));
+    }
+    return ST_OK;
+}
+
+int st_mg_smooth(st_ctx *c, int l, int color, const double *b, const double *xin, double *xout)
+{
+    if (!c || l < 0 || l >= c->L || !b || !xin || !xout) return ST_EINVAL;
+    if (c->dirty) CKR(setup_hierarchy(c));
+    const Lvl &g = c->lv[l];
+    long nd = level_dofs(c, l);
+    CKR(stage_in(c, b, nd));
+    launch_pack<double>(g, c->abi_a, c->BA, 0, c->fl[l], 1.0, c->s);
+    CKR(stage_in(c, xin, nd));
+    launch_pack<double>(g, c->abi_a, c->Wv, 0, c->fl[l], 1.0, c->s);
+    launch_fill<double>((long)g.nfield * g.S, c->LAM, 0.0, c->s);
+    launch_smooth<double>(g, c->fl[l], c->h64.al[l], c->BA, c->Wv, c->LAM, color, (float)c->prm.omega, false, c->s);
+    launch_unpack<double>(g, c->LAM, c->abi_b, c->s);
+    return stage_out(c, xout, nd);
+}
+
+int st_mg_restrict(st_ctx *c, int l, const double *rf, double *bc)
+{
+    if (!c || l < 0 || l + 1 >= c->L || !rf || !bc) return ST_EINVAL;
+    if (c->dirty) CKR(setup_hierarchy(c));
+    const Lvl &g = c->lv[l], &gc = c->lv[l + 1];
+    CKR(stage_in(c, rf, level_dofs(c, l)));
+    launch_pack<double>(g, c->abi_a, c->BA, 0, c->fl[l], 1.0, c->s);
+    launch_restrict<double>(g, gc, c->fl[l + 1], c->BA, c->Wv, c->s);
+    launch_unpack<double>(gc, c->Wv, c->abi_b, c->s);
+    return stage_out(c, bc, level_dofs(c, l + 1));
+}
+
+int st_mg_prolong(st_ctx *c, int l, const double *ec, const double *xf, double *xf_out)
+{
+    if (!c || l < 0 || l + 1 >= c->L || !ec || !xf || !xf_out) return ST_EINVAL;
+    if (c->dirty) CKR(setup_hierarchy(c));
+    const Lvl &g = c->lv[l], &gc = c->lv[l + 1];
+    CKR(stage_in(c, ec, level_dofs(c, l + 1)));
+    launch_pack<double>(gc, c->abi_a, c->BA, 0, c->fl[l + 1], 1.0, c->s);
+    CKR(stage_in(c, xf, level_dofs(c, l)));
+    launch_pack<double>(g, c->abi_a, c->Wv, 0, c->fl[l], 1.0, c->s);
+    launch_prolong<double>(g, gc, c->fl[l], c->BA, c->Wv, c->s);
+    launch_unpack<double>(g, c->Wv, c->abi_b, c->s);
+    return stage_out(c, xf_out, level_dofs(c, l));
+}
+
+}  // extern "C"

@@ -1,0 +1,158 @@
+"""GPU parity: every HIP kernel path against the CPU FP64 oracle, through the C ABI.
+
+Tolerances (FP64 paths): the stencils sum the same terms in a different order than
+the oracle's separate DEC passes, so results agree to rounding: 1e-12 relative.
+Solves are compared after both reach a 1e-10 true relative residual.
+"""
+import numpy as np
+import pytest
+
+from helpers import oracle_desc, random_dofs, setup, solver_for
+from conftest import rel
+
+pytestmark = pytest.mark.gpu
+
+CASES = [(3, 8), (3, 16), (2, 16), (2, 32)]
+
+
+@pytest.mark.parametrize("dim,n", CASES)
+def test_apply_matches_oracle(make_solver, orc, dim, n):
+    P, rho = setup(dim, n)
+    s = solver_for(make_solver, P, rho)
+    fixed, vol, pen = oracle_desc(orc, P, rho)
+    for seed in range(2):
+        x = random_dofs(P, seed)
+        y = s.applyMapping(x)
+        yo = orc.apply(dim, P.shape, fixed, vol, pen, x)
+        assert rel(y, yo) < 1e-12
+
+
+@pytest.mark.parametrize("dim,n", CASES)
+def test_design_alpha_matches_oracle(make_solver, orc, dim, n):
+    P, rho = setup(dim, n)
+    s = solver_for(make_solver, P, rho)
+    fixed, vol, pen = oracle_desc(orc, P, rho)
+    f0, a0 = s.mg_level_desc(0)
+    nc = orc.sizes(dim, *P.shape)[1][0]
+    assert np.array_equal(f0, P.fixed)
+    assert rel(a0, pen[nc:]) < 1e-14
+
+
+@pytest.mark.parametrize("dim,n", CASES)
+def test_objective_and_adjoint_rhs(make_solver, orc, dim, n):
+    P, rho = setup(dim, n)
+    s = solver_for(make_solver, P, rho)
+    fixed, vol, pen = oracle_desc(orc, P, rho)
+    x = random_dofs(P, 3)
+    J = s.compute_energy(x)
+    Jo = orc.objective(dim, P.shape, fixed, vol, pen, x)
+    assert abs(J - Jo) <= 1e-12 * abs(Jo)
+    g = s.compute_gradient_x(x)
+    go = orc.energy_op(dim, P.shape, fixed, vol, pen, x)
+    assert rel(g, go) < 1e-12
+
+
+@pytest.mark.parametrize("dim,n", CASES)
+def test_sensitivity_matches_oracle(make_solver, orc, dim, n):
+    P, rho = setup(dim, n)
+    s = solver_for(make_solver, P, rho)
+    nc = orc.sizes(dim, *P.shape)[1][0]
+    x = random_dofs(P, 4)
+    lam = random_dofs(P, 5, dirichlet=False)
+    dj = s.sensitivity(rho, x, lam)
+    djo = orc.sensitivity(dim, P.shape, rho, P.alpha.array(), P.fixed[nc:], x[nc:], lam[nc:])
+    assert rel(dj, djo) < 1e-13
+
+
+@pytest.mark.parametrize("dim,n", CASES)
+def test_filter_matches_oracle(make_solver, orc, dim, n):
+    P, rho = setup(dim, n)
+    s = solver_for(make_solver, P, rho)
+    v = np.random.default_rng(1).random(rho.size)
+    for tr in (False, True):
+        assert rel(s.filter(v, tr), orc.filt(dim, P.shape, 1.5, v, tr)) < 1e-14
+
+
+@pytest.mark.parametrize("dim,n", [(3, 8), (2, 32)])
+def test_oc_matches_oracle(make_solver, orc, dim, n):
+    P, rho = setup(dim, n)
+    s = solver_for(make_solver, P, rho)
+    dc = -np.random.default_rng(2).random(rho.size) * 1e-2
+    r1, c1 = s.oc_update(rho, dc, 0.3)
+    r2, c2 = orc.oc_update(dim, P.shape, 1.5, rho, dc, 0.3)
+    assert np.max(np.abs(r1 - r2)) < 1e-9
+    assert abs(c1 - c2) < 1e-9
+
+
+@pytest.mark.parametrize("dim,n", [(3, 16), (2, 32)])
+def test_mg_components_match_oracle(make_solver, orc, dim, n):
+    P, rho = setup(dim, n)
+    s = solver_for(make_solver, P, rho)
+    fixed, vol, pen = oracle_desc(orc, P, rho)
+    mg = orc.MG(dim, P.shape, fixed, pen, nu_pre=2, nu_post=2, omega=1.0, tang=0)
+    assert s.mg_levels() == mg.levels
+    rng = np.random.default_rng(7)
+    for l in range(mg.levels):
+        fo, po = mg.level_desc(l)
+        f, a = s.mg_level_desc(l)
+        nl = mg.level_dims(l)[0]
+        nc = orc.sizes(dim, nl, nl, nl if dim == 3 else 1)[1][0]
+        assert np.array_equal(f, fo), f"flags level {l}"
+        assert rel(a, po[nc:]) < 1e-13, f"alpha level {l}"
+        shp = (nl, nl, nl if dim == 3 else 1)
+        b = rng.standard_normal(fo.size)
+        xin = rng.standard_normal(fo.size)
+        vol_l = np.where(np.arange(fo.size) >= nc, 1.0 - fo, 0.0)
+        for color in (0, 1):
+            xo = orc.smooth_color(dim, shp, fo, po, b, xin, color, 1.0)
+            xg = s.mg_smooth(l, color, b, xin)
+            assert rel(xg, xo) < 1e-12, f"smooth level {l} colour {color}"
+        if l + 1 < mg.levels:
+            fc, pc = mg.level_desc(l + 1)
+            bo = orc.restrict(dim, shp, fc, 0, b)
+            assert rel(s.mg_restrict(l, b), bo) < 1e-13
+            ec = rng.standard_normal(fc.size)
+            xo = orc.prolong_add(dim, shp, fo, 0, ec, xin)
+            assert rel(s.mg_prolong(l, ec, xin), xo) < 1e-13
+
+
+@pytest.mark.parametrize("dim,n", [(3, 16), (2, 32)])
+def test_vcycle_matches_oracle(make_solver, orc, dim, n):
+    P, rho = setup(dim, n)
+    s = solver_for(make_solver, P, rho)
+    fixed, vol, pen = oracle_desc(orc, P, rho)
+    mg = orc.MG(dim, P.shape, fixed, pen, nu_pre=2, nu_post=2, omega=1.0, tang=0)
+    b = orc.lift_rhs(dim, P.shape, fixed, P.dirichlet)
+    z = s.mg_vcycle(b)
+    zo = mg.vcycle(b)
+    assert rel(z, zo) < 1e-10
+
+
+@pytest.mark.parametrize("dim,n", [(3, 16), (2, 32)])
+def test_solve_matches_oracle(make_solver, orc, dim, n):
+    P, rho = setup(dim, n)
+    s = solver_for(make_solver, P, rho, rtol=1e-10)
+    fixed, vol, pen = oracle_desc(orc, P, rho)
+    x, st = s.solve()
+    assert st["converged"] and st["relres"] <= 1e-10
+    b = orc.lift_rhs(dim, P.shape, fixed, P.dirichlet)
+    # the true residual recomputed by the oracle's operator
+    r = b - orc.apply(dim, P.shape, fixed, vol, pen, x)
+    assert np.linalg.norm(r) / np.linalg.norm(b) < 2e-10
+    mg = orc.MG(dim, P.shape, fixed, pen, nu_pre=2, nu_post=2, omega=1.0, tang=0)
+    xo, it, rr, _ = mg.fgmres(b, rtol=1e-12, restart=30, maxit=100)
+    assert rr < 1e-12
+    assert rel(x, xo) < 1e-7
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16"])
+@pytest.mark.parametrize("dim,n", [(3, 16), (2, 32)])
+def test_mixed_precision_solve(make_solver, orc, dim, n, prec):
+    P, rho = setup(dim, n)
+    s = solver_for(make_solver, P, rho, rtol=1e-8, precision=prec)
+    fixed, vol, pen = oracle_desc(orc, P, rho)
+    x, st = s.solve()
+    assert st["converged"] and st["relres"] <= 1e-8
+    b = orc.lift_rhs(dim, P.shape, fixed, P.dirichlet)
+    r = b - orc.apply(dim, P.shape, fixed, vol, pen, x)
+    assert np.linalg.norm(r) / np.linalg.norm(b) < 1.5e-8
