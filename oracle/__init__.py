@@ -255,8 +255,8 @@ def coarsen(dim, fshape, ffixed, fpen):
 class MG:
     """The multigrid preconditioner and FGMRES driver, restated on the CPU."""
 
-    def __init__(self, dim, shape, fixed, pen, nu_pre=2, nu_post=2, omega=0.8, tang=1, closed=0,
-                 coarsest=4):
+    def __init__(self, dim, shape, fixed, pen, nu_pre=2, nu_post=2, omega=1.0, tang=0, closed=0,
+                 coarsest=2):
         nx, ny, nz = shape
         self.dim, self.shape = dim, shape
         self._fixed = _b(fixed)
@@ -303,3 +303,29 @@ class MG:
         it = lib().orc_fgmres(self.h, _d(_f64(b)), _d(x), float(rtol), int(restart), int(maxit),
                               ctypes.byref(rr), _d(hist))
         return x, it, rr.value, hist[: it + 1]
+
+
+def to_step(P, rho, radius=1.5, volfrac=0.3, move=0.2, eta=0.5, rho_min=1e-3, rtol=1e-10, x_prev=None):
+    """One density-filtered OC design iteration, composed from the oracle pieces in the
+    order of FT/FluidTopoOpt3D.cpp:4-17 (objective -> gradient via adjoint -> update).
+    Returns (rho_new, J, change, x, lam)."""
+    dim, shape = P.dim, P.shape
+    n, s = sizes(dim, *shape)
+    nc = s[0]
+    rhof = filt(dim, shape, radius, rho) if radius > 1.0 else rho.copy()
+    am = P.alpha.array()
+    fa = design_to_alpha(dim, shape, rhof, am, P.fixed[nc:])
+    pen = np.zeros(n)
+    pen[nc:] = fa
+    vol = np.zeros(n)
+    vol[nc:] = 1.0 - P.fixed[nc:]
+    b = lift_rhs(dim, shape, P.fixed, P.dirichlet)
+    mg = MG(dim, shape, P.fixed, pen)
+    x, _, rr, _ = mg.fgmres(b, x0=P.dirichlet if x_prev is None else x_prev, rtol=rtol)
+    J = objective(dim, shape, P.fixed, vol, pen, x)
+    gx = energy_op(dim, shape, P.fixed, vol, pen, x)
+    lam, _, _, _ = mg.fgmres(gx, rtol=rtol)
+    dj = sensitivity(dim, shape, rhof, am, P.fixed[nc:], x[nc:], lam[nc:])
+    dc = filt(dim, shape, radius, dj, transpose=True) if radius > 1.0 else dj
+    rn, ch = oc_update(dim, shape, radius, rho, dc, volfrac, move, eta, rho_min)
+    return rn, J, ch, x, lam

@@ -776,7 +776,7 @@ orc_mg *orc_mg_create(int dim, int nx, int ny, int nz, const uint8_t *fixed, con
     mg->nu_pre = (int)prm[0]; mg->nu_post = (int)prm[1]; mg->omega = prm[2];
     mg->tang = (int)prm[3]; mg->closed = (int)prm[4];
     int coarsest = (int)prm[5];
-    if (coarsest < 2) coarsest = 4;
+    if (coarsest < 2) coarsest = 2;
     G g = mkg(dim, nx, ny, nz);
     int L = 0;
     for (;;) {

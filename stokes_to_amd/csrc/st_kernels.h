@@ -62,4 +62,15 @@ void launch_multiaxpy(long n, TS *w, const TS *V, long ld, int m, const double *
 template <typename TX, typename TY> void launch_axpby(long n, double a, const TX *x, double b, TY *y, hipStream_t);
 template <typename TS> void launch_fill(long n, TS *y, double v, hipStream_t);
 
+// 2.5D marching kernels (st_march.hip) for 3D levels with ny % 4 == 0, nz % 64 == 0
+bool march_ok(const Lvl &);
+unsigned march_blocks(const Lvl &);
+template <typename TS> void march_apply(const Lvl &, const uint8_t *, const TS *, const TS *, TS *, hipStream_t);
+template <typename TS>
+void march_residual(const Lvl &, const uint8_t *, const TS *, const TS *b, const TS *x, TS *r, double *part,
+                    hipStream_t);
+template <typename TS>
+void march_smooth(const Lvl &, const uint8_t *, const TS *, const TS *b, const TS *xin, TS *xout, int color,
+                  float omega, hipStream_t);
+
 }  // namespace st

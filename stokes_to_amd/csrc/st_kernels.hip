@@ -915,6 +915,7 @@ static inline unsigned nb_grid(long n)
 template <typename TS>
 void launch_apply(const Lvl &g, const uint8_t *fl, const TS *al, const TS *x, TS *y, hipStream_t s)
 {
+    if (march_ok(g)) { march_apply<TS>(g, fl, al, x, y, s); return; }
     if (g.dim == 3) hipLaunchKernelGGL((k_apply<TS, 3>), dim3(nb(g.S)), dim3(256), 0, s, g, fl, al, x, y);
     else hipLaunchKernelGGL((k_apply<TS, 2>), dim3(nb(g.S)), dim3(256), 0, s, g, fl, al, x, y);
 }
@@ -922,16 +923,18 @@ template <typename TS>
 void launch_residual(const Lvl &g, const uint8_t *fl, const TS *al, const TS *b, const TS *x, TS *r, double *part,
                      hipStream_t s)
 {
+    if (march_ok(g)) { march_residual<TS>(g, fl, al, b, x, r, part, s); return; }
     if (g.dim == 3) hipLaunchKernelGGL((k_residual<TS, 3>), dim3(nb(g.S)), dim3(256), 0, s, g, fl, al, b, x, r, part);
     else hipLaunchKernelGGL((k_residual<TS, 2>), dim3(nb(g.S)), dim3(256), 0, s, g, fl, al, b, x, r, part);
 }
-unsigned residual_blocks(const Lvl &g) { return nb(g.S); }
+unsigned residual_blocks(const Lvl &g) { return march_ok(g) ? march_blocks(g) : nb(g.S); }
 
 template <typename TS>
 void launch_smooth(const Lvl &g, const uint8_t *fl, const TS *al, const TS *b, const TS *xin, TS *xout, int color,
                    float omega, bool zero, hipStream_t s)
 {
     long nc = (long)g.nx * g.ny * g.nz;
+    if (!zero && march_ok(g)) { march_smooth<TS>(g, fl, al, b, xin, xout, color, omega, s); return; }
     if (g.dim == 3) {
         if (zero) hipLaunchKernelGGL((k_smooth<TS, 3, true>), dim3(nb(nc)), dim3(256), 0, s, g, fl, al, b, xin, xout, color, omega);
         else hipLaunchKernelGGL((k_smooth<TS, 3, false>), dim3(nb(nc)), dim3(256), 0, s, g, fl, al, b, xin, xout, color, omega);

@@ -1,0 +1,323 @@
+// 2.5D "marching" kernels for the fine 3D levels (gfx950).
+//
+// A workgroup of 4 x 64 threads owns a (j, k) tile of node columns and walks a
+// chunk of i-planes.  For each plane it stages, once, in LDS:
+//   w = m u (masked face velocities), p, flags          3 rolling planes
+//   edge curls e = Cod1 w   (CX/grid3D.cu:469-568)       2 rolling planes
+//   g = m_c D2 w            (CX/grid3D.cu:330-390)       2 rolling planes
+// so every stencil term of ST/StokeFlowMapping3D.cu:176-306 is formed once per
+// node and read from LDS by its consumers instead of being re-derived from
+// global memory (what the simple gather kernels of st_kernels.hip do).
+// The k direction (contiguous in memory) maps to the 64 lanes of a wave, so all
+// global loads and stores are coalesced.
+//
+// Modes: APPLY y = A x;  RESID r = b - A x (+ FP64 partial sums of r^2);
+//        SMOOTH one colour of the box-Vanka smoother (xin -> xout), with the
+//        7 local residuals of each colour cell read from the staged planes.
+#include "st_kernels.h"
+
+namespace st {
+
+namespace {
+constexpr int BJ = 4, BK = 64, LJ = BJ + 3, LK = BK + 3, NL = LJ * LK;
+constexpr int NT = BJ * BK;
+enum { M_APPLY = 0, M_RESID = 1, M_SMOOTH = 2 };
+
+__device__ __forceinline__ int slot3(int s) { return (s + 3) % 3; }
+__device__ __forceinline__ int slot2(int s) { return s & 1; }
+
+__device__ __forceinline__ double wsum64(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    return v;
+}
+}  // namespace
+
+template <typename TS, int MODE>
+__global__ void __launch_bounds__(256) k_march(Lvl g, const uint8_t *__restrict__ fl, const TS *__restrict__ al,
+                                               const TS *__restrict__ b, const TS *__restrict__ x,
+                                               TS *__restrict__ y, double *part, int chunk, int color, float omega)
+{
+    typedef typename Cvt<TS>::C C;
+    __shared__ C sW[3][3][NL];
+    __shared__ C sP[3][NL];
+    __shared__ uint8_t sF[3][NL];
+    __shared__ C sE[2][3][NL];
+    __shared__ C sG[2][NL];
+    __shared__ double sred[NT / 64];
+
+    const int tid = threadIdx.x;
+    const int tk = tid & (BK - 1), tj = tid / BK;
+    const int k0 = blockIdx.x * BK, j0 = blockIdx.y * BJ;
+    const int j = j0 + tj, k = k0 + tk;
+    const int ilast = (MODE == M_SMOOTH) ? g.nx : g.nx + 1;  // output planes [0, ilast)
+    const int ic0 = blockIdx.z * chunk;
+    int ic1 = (blockIdx.z == gridDim.z - 1) ? ilast : ic0 + chunk;
+    if (ic1 > ilast) ic1 = ilast;
+    const long S = g.S;
+    const long sI = (long)g.Py * g.Pz, sJ = g.Pz;
+    const int cown = (tj + 1) * LK + (tk + 1);
+
+    auto load_plane = [&](int s) {
+        const int sl = slot3(s);
+        for (int t = tid; t < NL; t += NT) {
+            const int jj = j0 - 1 + t / LK, kk = k0 - 1 + t % LK;
+            C wx = C(0), wy = C(0), wz = C(0), pp = C(0);
+            uint8_t f = 0;
+            if (s >= 0 && s <= g.nx && jj >= 0 && jj <= g.ny && kk >= 0 && kk <= g.nz) {
+                const long q = (long)s * sI + (long)jj * sJ + kk;
+                f = fl[q];
+                pp = Cvt<TS>::ld(x[q]);
+                if (f & 2) wx = Cvt<TS>::ld(x[S + q]);
+                if (f & 4) wy = Cvt<TS>::ld(x[2 * S + q]);
+                if (f & 8) wz = Cvt<TS>::ld(x[3 * S + q]);
+            }
+            sW[sl][0][t] = wx;
+            sW[sl][1][t] = wy;
+            sW[sl][2][t] = wz;
+            sP[sl][t] = pp;
+            sF[sl][t] = f;
+        }
+    };
+    // edge curls of plane s on [j0, j0+BJ] x [k0, k0+BK]
+    auto edges = [&](int s) {
+        const int ws = slot3(s), wm = slot3(s - 1), es = slot2(s);
+        for (int t = tid; t < (BJ + 1) * (BK + 1); t += NT) {
+            const int c = (t / (BK + 1) + 1) * LK + (t % (BK + 1) + 1);
+            sE[es][0][c] = sW[ws][1][c] - sW[ws][1][c - 1] - sW[ws][2][c] + sW[ws][2][c - LK];
+            sE[es][1][c] = sW[ws][2][c] - sW[wm][2][c] - sW[ws][0][c] + sW[ws][0][c - 1];
+            sE[es][2][c] = sW[ws][0][c] - sW[ws][0][c - LK] - sW[ws][1][c] + sW[wm][1][c];
+        }
+    };
+    // g = m_c div(w) of plane s on [j0-1, j0+BJ] x [k0-1, k0+BK]
+    auto gdiv = [&](int s) {
+        const int ws = slot3(s), wn = slot3(s + 1), gs = slot2(s);
+        for (int t = tid; t < (BJ + 2) * (BK + 2); t += NT) {
+            const int c = (t / (BK + 2)) * LK + (t % (BK + 2));
+            const C dv = sW[wn][0][c] - sW[ws][0][c] + sW[ws][1][c + LK] - sW[ws][1][c] + sW[ws][2][c + 1] -
+                         sW[ws][2][c];
+            sG[gs][c] = (sF[ws][c] & 1) ? dv : C(0);
+        }
+    };
+
+    // ---- preamble for the chunk
+    load_plane(ic0 - 1);
+    load_plane(ic0);
+    load_plane(ic0 + 1);
+    __syncthreads();
+    edges(ic0);
+    gdiv(ic0);
+    C gm1, pm1;
+    int fm1;
+    {
+        const int a = slot3(ic0 - 1), bq = slot3(ic0);
+        const C dv = sW[bq][0][cown] - sW[a][0][cown] + sW[a][1][cown + LK] - sW[a][1][cown] + sW[a][2][cown + 1] -
+                     sW[a][2][cown];
+        fm1 = sF[a][cown];
+        gm1 = (fm1 & 1) ? dv : C(0);
+        pm1 = sP[a][cown];
+    }
+    __syncthreads();
+
+    double acc = 0.0;
+    for (int i = ic0; i < ic1; i++) {
+        load_plane(i + 2);
+        __syncthreads();
+        edges(i + 1);
+        gdiv(i + 1);
+        __syncthreads();
+        const int ps = slot3(i), pn = slot3(i + 1), es = slot2(i), en = slot2(i + 1), gs = slot2(i), gn = slot2(i + 1);
+        const long q = (long)i * sI + (long)j * sJ + k;
+
+        // operator row values at plane i, LDS position c (node (i, j0+.., k0+..))
+        auto yp = [&](int c, long qq) -> C { return (sF[ps][c] & 1) ? sG[gs][c] : sP[ps][c]; };
+        auto yx = [&](int c, long qq, C gim1, C pim1) -> C {
+            if (!(sF[ps][c] & 2)) return Cvt<TS>::ld(x[S + qq]);
+            C r = gim1 - sG[gs][c] + sE[es][1][c + 1] - sE[es][1][c] + sE[es][2][c] - sE[es][2][c + LK];
+            return r + pim1 - sP[ps][c] + Cvt<TS>::ld(al[qq]) * sW[ps][0][c];
+        };
+        auto yy = [&](int c, long qq) -> C {
+            if (!(sF[ps][c] & 4)) return Cvt<TS>::ld(x[2 * S + qq]);
+            C r = sG[gs][c - LK] - sG[gs][c] + sE[es][0][c] - sE[es][0][c + 1] + sE[en][2][c] - sE[es][2][c];
+            return r + sP[ps][c - LK] - sP[ps][c] + Cvt<TS>::ld(al[S + qq]) * sW[ps][1][c];
+        };
+        auto yz = [&](int c, long qq) -> C {
+            if (!(sF[ps][c] & 8)) return Cvt<TS>::ld(x[3 * S + qq]);
+            C r = sG[gs][c - 1] - sG[gs][c] + sE[es][0][c + LK] - sE[es][0][c] + sE[es][1][c] - sE[en][1][c];
+            return r + sP[ps][c - 1] - sP[ps][c] + Cvt<TS>::ld(al[2 * S + qq]) * sW[ps][2][c];
+        };
+
+        if (MODE != M_SMOOTH) {
+            auto emit = [&](int c, long qq, C gim1, C pim1) {
+                C v0 = yp(c, qq), v1 = yx(c, qq, gim1, pim1), v2 = yy(c, qq), v3 = yz(c, qq);
+                if (MODE == M_RESID) {
+                    v0 = Cvt<TS>::ld(b[qq]) - v0;
+                    v1 = Cvt<TS>::ld(b[S + qq]) - v1;
+                    v2 = Cvt<TS>::ld(b[2 * S + qq]) - v2;
+                    v3 = Cvt<TS>::ld(b[3 * S + qq]) - v3;
+                    acc += (double)v0 * v0 + (double)v1 * v1 + (double)v2 * v2 + (double)v3 * v3;
+                }
+                y[qq] = Cvt<TS>::st(v0);
+                y[S + qq] = Cvt<TS>::st(v1);
+                y[2 * S + qq] = Cvt<TS>::st(v2);
+                y[3 * S + qq] = Cvt<TS>::st(v3);
+            };
+            emit(cown, q, gm1, pm1);
+            // the padded node column k = nz and row j = ny belong to the last tiles
+            const bool lastk = (k == g.nz - 1), lastj = (j == g.ny - 1);
+            if (lastk) emit(cown + 1, q + 1, C(0), C(0));
+            if (lastj) emit(cown + LK, q + sJ, C(0), C(0));
+            if (lastk && lastj) emit(cown + LK + 1, q + sJ + 1, C(0), C(0));
+        } else {
+            // ---- one colour of the box-Vanka smoother for the cells of plane i
+            const int c = cown;
+            const bool mine = ((i + j + k) & 1) == color;
+            const int fb = sF[ps][c];
+            if (!mine) {
+                y[q] = Cvt<TS>::st(sP[ps][c]);
+                if (i == 0) y[S + q] = x[S + q];
+                if (i == g.nx - 1) y[S + q + sI] = x[S + q + sI];
+                if (j == 0) y[2 * S + q] = x[2 * S + q];
+                if (j == g.ny - 1) y[2 * S + q + sJ] = x[2 * S + q + sJ];
+                if (k == 0) y[3 * S + q] = x[3 * S + q];
+                if (k == g.nz - 1) y[3 * S + q + 1] = x[3 * S + q + 1];
+            } else {
+                const C mc = (fb & 1) ? C(1) : C(0);
+                const long qh[3] = {q + sI, q + sJ, q + 1};
+                // residuals at the 7 DOFs
+                const C rp = Cvt<TS>::ld(b[q]) - yp(c, q);
+                C r0[3], r1[3], u0[3], u1[3], m0[3], m1[3], a0[3], a1[3];
+                // low faces (node i,j,k)
+                r0[0] = Cvt<TS>::ld(b[S + q]) - yx(c, q, gm1, pm1);
+                r0[1] = Cvt<TS>::ld(b[2 * S + q]) - yy(c, q);
+                r0[2] = Cvt<TS>::ld(b[3 * S + q]) - yz(c, q);
+                // high x face: node (i+1, j, k), from plane i+1 (slots pn, en, gn)
+                {
+                    const int f1 = sF[pn][c];
+                    C v;
+                    if (!(f1 & 2)) v = Cvt<TS>::ld(x[S + qh[0]]);
+                    else {
+                        v = sG[gs][c] - sG[gn][c] + sE[en][1][c + 1] - sE[en][1][c] + sE[en][2][c] - sE[en][2][c + LK];
+                        v += sP[ps][c] - sP[pn][c] + Cvt<TS>::ld(al[qh[0]]) * sW[pn][0][c];
+                    }
+                    r1[0] = Cvt<TS>::ld(b[S + qh[0]]) - v;
+                }
+                r1[1] = Cvt<TS>::ld(b[2 * S + qh[1]]) - yy(c + LK, qh[1]);
+                r1[2] = Cvt<TS>::ld(b[3 * S + qh[2]]) - yz(c + 1, qh[2]);
+                // masks, raw values, diagonals
+                const int fl0[3] = {fb, fb, fb};
+                const int fl1[3] = {sF[pn][c], sF[ps][c + LK], sF[ps][c + 1]};
+                const C clo[3] = {(C)(fm1 & 1), (C)(sF[ps][c - LK] & 1), (C)(sF[ps][c - 1] & 1)};
+                const C chi[3] = {(C)(sF[pn][c] & 1), (C)(sF[ps][c + LK] & 1), (C)(sF[ps][c + 1] & 1)};
+                const int cw1[3] = {c, c + LK, c + 1};
+#pragma unroll
+                for (int d = 0; d < 3; d++) {
+                    m0[d] = ((fl0[d] >> (1 + d)) & 1) ? C(1) : C(0);
+                    m1[d] = ((fl1[d] >> (1 + d)) & 1) ? C(1) : C(0);
+                    u0[d] = (m0[d] != C(0)) ? sW[ps][d][c] : Cvt<TS>::ld(x[(long)(1 + d) * S + q]);
+                    const int slot1 = (d == 0) ? pn : ps;
+                    u1[d] = (m1[d] != C(0)) ? sW[slot1][d][cw1[d]] : Cvt<TS>::ld(x[(long)(1 + d) * S + qh[d]]);
+                    const C al0 = Cvt<TS>::ld(al[(long)d * S + q]), al1 = Cvt<TS>::ld(al[(long)d * S + qh[d]]);
+                    a0[d] = m0[d] * (clo[d] + mc + C(4)) + al0 + (C(1) - m0[d]);
+                    a1[d] = m1[d] * (mc + chi[d] + C(4)) + al1 + (C(1) - m1[d]);
+                }
+                C Sc = C(1) - mc, tp = rp;
+                C of[3];
+#pragma unroll
+                for (int d = 0; d < 3; d++) {
+                    of[d] = -m0[d] * mc * m1[d];
+                    const C id = C(1) / (a0[d] * a1[d] - of[d] * of[d]);
+                    const C kb0 = id * (a1[d] * (-m0[d]) - of[d] * m1[d]);
+                    const C kb1 = id * (-of[d] * (-m0[d]) + a0[d] * m1[d]);
+                    const C kr0 = id * (a1[d] * r0[d] - of[d] * r1[d]);
+                    const C kr1 = id * (-of[d] * r0[d] + a0[d] * r1[d]);
+                    Sc -= mc * ((-m0[d]) * kb0 + m1[d] * kb1);
+                    tp -= mc * ((-m0[d]) * kr0 + m1[d] * kr1);
+                }
+                const C dp = (Sc != C(0)) ? tp / Sc : C(0);
+                const C om = (C)omega;
+                y[q] = Cvt<TS>::st(sP[ps][c] + om * dp);
+#pragma unroll
+                for (int d = 0; d < 3; d++) {
+                    const C id = C(1) / (a0[d] * a1[d] - of[d] * of[d]);
+                    const C e0 = r0[d] + m0[d] * dp, e1 = r1[d] - m1[d] * dp;
+                    y[(long)(1 + d) * S + q] = Cvt<TS>::st(u0[d] + om * id * (a1[d] * e0 - of[d] * e1));
+                    y[(long)(1 + d) * S + qh[d]] = Cvt<TS>::st(u1[d] + om * id * (-of[d] * e0 + a0[d] * e1));
+                }
+            }
+        }
+        gm1 = sG[gs][cown];
+        pm1 = sP[ps][cown];
+        fm1 = sF[ps][cown];
+        __syncthreads();
+    }
+    if (MODE == M_RESID && part) {
+        double v = wsum64(acc);
+        if ((tid & 63) == 0) sred[tid >> 6] = v;
+        __syncthreads();
+        if (tid == 0) {
+            double s = 0.0;
+            for (int w = 0; w < NT / 64; w++) s += sred[w];
+            part[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] = s;
+        }
+    }
+}
+
+bool march_ok(const Lvl &g) { return g.dim == 3 && g.nz % BK == 0 && g.ny % BJ == 0 && g.nx >= 8; }
+
+static dim3 march_grid(const Lvl &g, int &chunk)
+{
+    const int tiles = (g.nz / BK) * (g.ny / BJ);
+    int nch = (2048 + tiles - 1) / tiles;      // aim at >= 2048 workgroups
+    int maxch = g.nx / 8;                      // at least 8 planes per chunk
+    if (nch > maxch) nch = maxch;
+    if (nch < 1) nch = 1;
+    chunk = (g.nx + nch - 1) / nch;
+    nch = (g.nx + chunk - 1) / chunk;
+    return dim3(g.nz / BK, g.ny / BJ, nch);
+}
+
+unsigned march_blocks(const Lvl &g)
+{
+    int ch;
+    dim3 d = march_grid(g, ch);
+    return d.x * d.y * d.z;
+}
+
+template <typename TS>
+void march_apply(const Lvl &g, const uint8_t *fl, const TS *al, const TS *x, TS *y, hipStream_t s)
+{
+    int ch;
+    dim3 grid = march_grid(g, ch);
+    hipLaunchKernelGGL((k_march<TS, M_APPLY>), grid, dim3(NT), 0, s, g, fl, al, (const TS *)nullptr, x, y,
+                       (double *)nullptr, ch, 0, 0.f);
+}
+template <typename TS>
+void march_residual(const Lvl &g, const uint8_t *fl, const TS *al, const TS *b, const TS *x, TS *r, double *part,
+                    hipStream_t s)
+{
+    int ch;
+    dim3 grid = march_grid(g, ch);
+    hipLaunchKernelGGL((k_march<TS, M_RESID>), grid, dim3(NT), 0, s, g, fl, al, b, x, r, part, ch, 0, 0.f);
+}
+template <typename TS>
+void march_smooth(const Lvl &g, const uint8_t *fl, const TS *al, const TS *b, const TS *xin, TS *xout, int color,
+                  float omega, hipStream_t s)
+{
+    int ch;
+    dim3 grid = march_grid(g, ch);
+    hipLaunchKernelGGL((k_march<TS, M_SMOOTH>), grid, dim3(NT), 0, s, g, fl, al, b, xin, xout, (double *)nullptr,
+                       ch, color, omega);
+}
+
+#define INST_M(T)                                                                                              \
+    template void march_apply<T>(const Lvl &, const uint8_t *, const T *, const T *, T *, hipStream_t);       \
+    template void march_residual<T>(const Lvl &, const uint8_t *, const T *, const T *, const T *, T *,       \
+                                    double *, hipStream_t);                                                   \
+    template void march_smooth<T>(const Lvl &, const uint8_t *, const T *, const T *, const T *, T *, int,    \
+                                  float, hipStream_t);
+INST_M(double)
+INST_M(float)
+INST_M(bf16)
+
+}  // namespace st

@@ -445,7 +445,7 @@ int st_create(st_ctx **out, const st_params *p)
     int m = n;
     for (;;) {
         c->lv.push_back(make_lvl(c->dim, m, m, m));
-        if (m % 2 != 0 || m / 2 < 4) break;
+        if (m % 2 != 0 || m / 2 < 2) break;  // coarsest level 2^dim cells
         m /= 2;
     }
     c->L = (int)c->lv.size();

@@ -156,3 +156,29 @@ def test_mixed_precision_solve(make_solver, orc, dim, n, prec):
     b = orc.lift_rhs(dim, P.shape, fixed, P.dirichlet)
     r = b - orc.apply(dim, P.shape, fixed, vol, pen, x)
     assert np.linalg.norm(r) / np.linalg.norm(b) < 1.5e-8
+
+
+@pytest.mark.parametrize("dim,n", [(3, 64)])
+def test_fine_level_tiled_kernels_match_oracle(make_solver, orc, dim, n):
+    """64^3 exercises the 2.5D tiled (marching) operator and smoother kernels."""
+    P, rho = setup(dim, n)
+    s = solver_for(make_solver, P, rho)
+    fixed, vol, pen = oracle_desc(orc, P, rho)
+    x = random_dofs(P, 11)
+    assert rel(s.applyMapping(x), orc.apply(dim, P.shape, fixed, vol, pen, x)) < 1e-12
+    b = random_dofs(P, 12, dirichlet=False)
+    for color in (0, 1):
+        xo = orc.smooth_color(dim, P.shape, fixed, pen, b, x, color, 1.0)
+        assert rel(s.mg_smooth(0, color, b, x), xo) < 1e-12
+
+
+@pytest.mark.parametrize("dim,n", [(3, 64)])
+def test_fine_level_solve_64(make_solver, orc, dim, n):
+    P, rho = setup(dim, n)
+    s = solver_for(make_solver, P, rho, rtol=1e-8)
+    fixed, vol, pen = oracle_desc(orc, P, rho)
+    x, st = s.solve()
+    assert st["converged"] and st["iters"] <= 15
+    b = orc.lift_rhs(dim, P.shape, fixed, P.dirichlet)
+    r = b - orc.apply(dim, P.shape, fixed, vol, pen, x)
+    assert np.linalg.norm(r) / np.linalg.norm(b) < 1.5e-8
