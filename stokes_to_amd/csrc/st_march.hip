@@ -48,36 +48,63 @@ __global__ void __launch_bounds__(256) k_march(Lvl g, const uint8_t *__restrict_
 
     const int tid = threadIdx.x;
     const int tk = tid & (BK - 1), tj = tid / BK;
-    const int k0 = blockIdx.x * BK, j0 = blockIdx.y * BJ;
+    // XCD-aware tile order: workgroups are dealt round-robin to the 8 XCDs, so map
+    // hardware id h to logical tile (h % 8) * (nb / 8) + h / 8 -- neighbouring tiles
+    // (which share halo planes) then run on the same XCD and meet in its L2
+    const int nbx = gridDim.x, nby = gridDim.y, nbz = gridDim.z;
+    const int nbt = nbx * nby * nbz;
+    const int hw = blockIdx.x + nbx * (blockIdx.y + nby * blockIdx.z);
+    const int lid = (nbt % 8 == 0) ? (hw % 8) * (nbt / 8) + hw / 8 : hw;
+    const int bx = lid % nbx, by = (lid / nbx) % nby, bz = lid / (nbx * nby);
+    const int k0 = bx * BK, j0 = by * BJ;
     const int j = j0 + tj, k = k0 + tk;
     const int ilast = (MODE == M_SMOOTH) ? g.nx : g.nx + 1;  // output planes [0, ilast)
-    const int ic0 = blockIdx.z * chunk;
-    int ic1 = (blockIdx.z == gridDim.z - 1) ? ilast : ic0 + chunk;
+    const int ic0 = bz * chunk;
+    int ic1 = (bz == nbz - 1) ? ilast : ic0 + chunk;
     if (ic1 > ilast) ic1 = ilast;
     const long S = g.S;
     const long sI = (long)g.Py * g.Pz, sJ = g.Pz;
     const int cown = (tj + 1) * LK + (tk + 1);
 
-    auto load_plane = [&](int s) {
-        const int sl = slot3(s);
-        for (int t = tid; t < NL; t += NT) {
+    // plane staging: global -> registers (fetch, issued one plane ahead) and
+    // registers -> LDS (store); each thread owns LDS entries tid and tid + NT
+    C rw[2][4];
+    uint8_t rf[2];
+    auto fetch = [&](int s) {
+#pragma unroll
+        for (int e = 0; e < 2; e++) {
+            const int t = tid + e * NT;
+            rf[e] = 0;
+            rw[e][0] = rw[e][1] = rw[e][2] = rw[e][3] = C(0);
             const int jj = j0 - 1 + t / LK, kk = k0 - 1 + t % LK;
-            C wx = C(0), wy = C(0), wz = C(0), pp = C(0);
-            uint8_t f = 0;
-            if (s >= 0 && s <= g.nx && jj >= 0 && jj <= g.ny && kk >= 0 && kk <= g.nz) {
+            if (t < NL && s >= 0 && s <= g.nx && jj >= 0 && jj <= g.ny && kk >= 0 && kk <= g.nz) {
                 const long q = (long)s * sI + (long)jj * sJ + kk;
-                f = fl[q];
-                pp = Cvt<TS>::ld(x[q]);
-                if (f & 2) wx = Cvt<TS>::ld(x[S + q]);
-                if (f & 4) wy = Cvt<TS>::ld(x[2 * S + q]);
-                if (f & 8) wz = Cvt<TS>::ld(x[3 * S + q]);
+                rf[e] = fl[q];
+                rw[e][0] = Cvt<TS>::ld(x[q]);
+                rw[e][1] = Cvt<TS>::ld(x[S + q]);
+                rw[e][2] = Cvt<TS>::ld(x[2 * S + q]);
+                rw[e][3] = Cvt<TS>::ld(x[3 * S + q]);
             }
-            sW[sl][0][t] = wx;
-            sW[sl][1][t] = wy;
-            sW[sl][2][t] = wz;
-            sP[sl][t] = pp;
-            sF[sl][t] = f;
         }
+    };
+    auto store = [&](int s) {
+        const int sl = slot3(s);
+#pragma unroll
+        for (int e = 0; e < 2; e++) {
+            const int t = tid + e * NT;
+            if (t < NL) {
+                const int f = rf[e];
+                sP[sl][t] = rw[e][0];
+                sW[sl][0][t] = (f & 2) ? rw[e][1] : C(0);
+                sW[sl][1][t] = (f & 4) ? rw[e][2] : C(0);
+                sW[sl][2][t] = (f & 8) ? rw[e][3] : C(0);
+                sF[sl][t] = (uint8_t)f;
+            }
+        }
+    };
+    auto load_plane = [&](int s) {
+        fetch(s);
+        store(s);
     };
     // edge curls of plane s on [j0, j0+BJ] x [k0, k0+BK]
     auto edges = [&](int s) {
@@ -117,11 +144,13 @@ __global__ void __launch_bounds__(256) k_march(Lvl g, const uint8_t *__restrict_
         gm1 = (fm1 & 1) ? dv : C(0);
         pm1 = sP[a][cown];
     }
+    fetch(ic0 + 2);
     __syncthreads();
 
     double acc = 0.0;
     for (int i = ic0; i < ic1; i++) {
-        load_plane(i + 2);
+        store(i + 2);
+        if (i + 1 < ic1) fetch(i + 3);
         __syncthreads();
         edges(i + 1);
         gdiv(i + 1);
@@ -258,7 +287,7 @@ __global__ void __launch_bounds__(256) k_march(Lvl g, const uint8_t *__restrict_
         if (tid == 0) {
             double s = 0.0;
             for (int w = 0; w < NT / 64; w++) s += sred[w];
-            part[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] = s;
+            part[hw] = s;
         }
     }
 }
