@@ -128,9 +128,12 @@ int st_oc_update(st_ctx *c, const double *rho, const double *dc, double volfrac,
                  double rho_min, double *rho_new, double *change);
 /* one full design iteration on the device, the body of Optimize_MMA's loop
    (FT/FluidTopoOpt3D.cpp:4-17): filter, design->alpha, forward solve, objective,
-   adjoint solve, sensitivity, filter transpose, OC update of the device design */
-int st_to_step(st_ctx *c, double volfrac, double move, double eta, double rho_min, double *J, double *vol,
-               double *change, st_stats *fwd, st_stats *adj);
+   adjoint solve, sensitivity, filter transpose, OC update of the device design.
+   init: ST_INIT_ZERO (both solves from zero, as the reference's GMRES), ST_INIT_KEEP
+   (previous forward / adjoint solutions), ST_INIT_LEARNED (previous solution plus the
+   learned coarse-grid correction; needs a trained predictor, 3D only) */
+int st_to_step(st_ctx *c, int init, double volfrac, double move, double eta, double rho_min, double *J,
+               double *vol, double *change, st_stats *fwd, st_stats *adj);
 
 /* ---- kernel timing (HIP events around each launch on the context stream) ---- */
 enum { ST_PROF_SMOOTH = 0, ST_PROF_APPLY = 1, ST_PROF_RESIDUAL = 2, ST_PROF_NKIND = 3 };
@@ -141,11 +144,28 @@ int st_profile(st_ctx *c, int on);
    precision applies) */
 int st_profile_read(st_ctx *c, int kind, double *total_ms, long *launches, double *bytes_per_launch);
 
-/* ---- learned initial guess (no counterpart in the reference) ---- */
-/* train the conv-net predictor on the (design, solution) pairs recorded by previous
-   st_to_step / st_record_sample calls; returns the final training loss */
+/* ---- learned initial guess (no counterpart in the reference, whose GMRES starts
+   from zero, CX/GMRES.cpp:29-40) ----
+   A 3-level 3D conv U-Net (16/32/64 channels, 3x3x3, BF16 weights, FP32 accumulate,
+   BF16 MFMA kernels) on the n/4 multigrid level maps (block-mean density, boundary
+   channels) to the coarse state (p, u).  3D only. */
+/* record the pair (density used for the current alpha, current device solution) */
 int st_record_sample(st_ctx *c);
+int st_num_samples(const st_ctx *c);
+/* Adam training on the recorded samples (batch 1); loss = mean squared error of the
+   scaled coarse state after the last epoch */
 int st_net_train(st_ctx *c, int epochs, double lr, double *loss);
+/* parameter access and a full forward/backward on host features/targets
+   (R^3 x 4 each, R = n/4), used by the gradient parity tests */
+long st_net_num_params(st_ctx *c);
+int st_net_get_params(st_ctx *c, float *params);
+int st_net_set_params(st_ctx *c, const float *params);
+int st_net_grad(st_ctx *c, const float *features, const float *targets, double *loss, float *grad, float *out);
+/* one convolution layer on host arrays, no context: mode 0 forward (+bias, relu),
+   1 input gradient of gout, 2 weight and bias gradients of (in, gout); channels-last,
+   R^3 grid, zero padding, ntap 27 (3x3x3) or 1 */
+int st_net_conv_test(int R, int ntap, int cin, int cout, int relu, int mode, const float *in, const float *w,
+                     const float *bias, const float *gout, float *out, float *gw, float *gb);
 
 /* ---- multigrid internals exposed for component parity tests ---- */
 int st_mg_levels(const st_ctx *c);

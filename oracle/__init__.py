@@ -69,6 +69,9 @@ def lib():
         L.orc_restrict.argtypes = [I, I, I, I, U8, I, D, D]
         L.orc_prolong_add.argtypes = [I, I, I, I, U8, I, D, D]
         L.orc_coarsen.argtypes = [I, I, I, I, U8, D, U8, D]
+        L.orc_conv3d.argtypes = [I, I, I, I, D, D, D, D, I]
+        L.orc_conv3d_bwd_data.argtypes = [I, I, I, I, D, D, D]
+        L.orc_conv3d_bwd_weight.argtypes = [I, I, I, I, D, D, D, D]
         L.orc_fgmres.restype = I
         L.orc_fgmres.argtypes = [ctypes.c_void_p, D, D, ctypes.c_double, I, I, D, D]
     return _lib
@@ -216,6 +219,27 @@ def oc_update(dim, shape, radius, rho, dc, volfrac, move=0.2, eta=0.5, rho_min=1
     lib().orc_oc_update(dim, nx, ny, nz, float(radius), _d(_f64(rho)), _d(_f64(dc)), float(volfrac),
                         float(move), float(eta), float(rho_min), _d(out), ctypes.byref(ch))
     return out, ch.value
+
+
+def conv3d(R, ntap, cin, cout, x, w, b=None, relu=False):
+    """FP64 convolution, channels-last (R^3 x cin) -> (R^3 x cout), zero padding."""
+    out = np.zeros(R ** 3 * cout)
+    bb = _f64(b) if b is not None else np.zeros(cout)
+    lib().orc_conv3d(R, ntap, cin, cout, _d(_f64(x)), _d(_f64(w)), _d(bb), _d(out), int(relu))
+    return out
+
+
+def conv3d_bwd_data(R, ntap, cin, cout, gout, w):
+    out = np.zeros(R ** 3 * cin)
+    lib().orc_conv3d_bwd_data(R, ntap, cin, cout, _d(_f64(gout)), _d(_f64(w)), _d(out))
+    return out
+
+
+def conv3d_bwd_weight(R, ntap, cin, cout, x, gout):
+    gw = np.zeros(ntap * cin * cout)
+    gb = np.zeros(cout)
+    lib().orc_conv3d_bwd_weight(R, ntap, cin, cout, _d(_f64(x)), _d(_f64(gout)), _d(gw), _d(gb))
+    return gw, gb
 
 
 def smooth_color(dim, shape, fixed, pen, b, xin, color, omega):

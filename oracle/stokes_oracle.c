@@ -947,3 +947,82 @@ int orc_fgmres(orc_mg *mg, const double *b, double *x, double rtol, int m, int m
     free(V); free(Z); free(H); free(cs); free(sn); free(gv); free(w); free(yv);
     return it;
 }
+
+/* ------------------------------------------------------- learned-part oracle */
+/* 3x3x3 (or 1x1x1) convolution on an R^3 grid, channels-last, zero padding.
+ * tap t = (a+1)*9 + (b+1)*3 + (c+1) for offsets (a,b,c) along (i,j,k).
+ * out[v][co] = bias[co] + sum_t sum_ci in[v+o_t][ci] W[t][ci][co]  (FP64) */
+void orc_conv3d(int R, int ntap, int Cin, int Cout, const double *in, const double *W, const double *bias,
+                double *out, int relu)
+{
+    const long nv = (long)R * R * R;
+    const int rr = (ntap == 27) ? 1 : 0;
+    OMP_FOR
+    for (long v = 0; v < nv; v++) {
+        int i = (int)(v / ((long)R * R)), j = (int)((v / R) % R), k = (int)(v % R);
+        for (int co = 0; co < Cout; co++) {
+            double s = bias ? bias[co] : 0.0;
+            for (int a = -rr; a <= rr; a++) for (int b = -rr; b <= rr; b++) for (int c = -rr; c <= rr; c++) {
+                int ii = i + a, jj = j + b, kk = k + c;
+                if (ii < 0 || jj < 0 || kk < 0 || ii >= R || jj >= R || kk >= R) continue;
+                int t = (ntap == 27) ? (a + 1) * 9 + (b + 1) * 3 + (c + 1) : 0;
+                const double *x = in + (((long)ii * R + jj) * R + kk) * Cin;
+                const double *w = W + (long)t * Cin * Cout + co;
+                for (int ci = 0; ci < Cin; ci++) s += x[ci] * w[(long)ci * Cout];
+            }
+            out[v * Cout + co] = (relu && s < 0) ? 0.0 : s;
+        }
+    }
+}
+/* gin[v][ci] = sum_t sum_co gout[v-o_t][co] W[t][ci][co] */
+void orc_conv3d_bwd_data(int R, int ntap, int Cin, int Cout, const double *gout, const double *W, double *gin)
+{
+    const long nv = (long)R * R * R;
+    const int rr = (ntap == 27) ? 1 : 0;
+    OMP_FOR
+    for (long v = 0; v < nv; v++) {
+        int i = (int)(v / ((long)R * R)), j = (int)((v / R) % R), k = (int)(v % R);
+        for (int ci = 0; ci < Cin; ci++) {
+            double s = 0.0;
+            for (int a = -rr; a <= rr; a++) for (int b = -rr; b <= rr; b++) for (int c = -rr; c <= rr; c++) {
+                int ii = i - a, jj = j - b, kk = k - c;
+                if (ii < 0 || jj < 0 || kk < 0 || ii >= R || jj >= R || kk >= R) continue;
+                int t = (ntap == 27) ? (a + 1) * 9 + (b + 1) * 3 + (c + 1) : 0;
+                const double *g = gout + (((long)ii * R + jj) * R + kk) * Cout;
+                const double *w = W + ((long)t * Cin + ci) * Cout;
+                for (int co = 0; co < Cout; co++) s += g[co] * w[co];
+            }
+            gin[v * Cin + ci] = s;
+        }
+    }
+}
+/* gW[t][ci][co] = sum_v in[v+o_t][ci] gout[v][co];  gb[co] = sum_v gout[v][co] */
+void orc_conv3d_bwd_weight(int R, int ntap, int Cin, int Cout, const double *in, const double *gout, double *gW,
+                           double *gb)
+{
+    const long nv = (long)R * R * R;
+    const int rr = (ntap == 27) ? 1 : 0;
+    OMP_FOR
+    for (int t = 0; t < ntap; t++) {
+        int a = (ntap == 27) ? t / 9 - 1 : 0, b = (ntap == 27) ? (t / 3) % 3 - 1 : 0, c = (ntap == 27) ? t % 3 - 1 : 0;
+        double *w = gW + (long)t * Cin * Cout;
+        for (long q = 0; q < (long)Cin * Cout; q++) w[q] = 0.0;
+        for (long v = 0; v < nv; v++) {
+            int i = (int)(v / ((long)R * R)), j = (int)((v / R) % R), k = (int)(v % R);
+            int ii = i + a, jj = j + b, kk = k + c;
+            if (ii < 0 || jj < 0 || kk < 0 || ii >= R || jj >= R || kk >= R) continue;
+            const double *x = in + (((long)ii * R + jj) * R + kk) * Cin;
+            const double *g = gout + v * Cout;
+            for (int ci = 0; ci < Cin; ci++)
+                for (int co = 0; co < Cout; co++) w[(long)ci * Cout + co] += x[ci] * g[co];
+        }
+    }
+    (void)rr;
+    if (gb) {
+        for (int co = 0; co < Cout; co++) {
+            double s = 0.0;
+            for (long v = 0; v < nv; v++) s += gout[v * Cout + co];
+            gb[co] = s;
+        }
+    }
+}

@@ -12,7 +12,8 @@
  * A "descriptor" (fixed / vol / pen) uses the same [cells | faces] order, as the
  * reference's StokeFlowDescriptor3D does (ST/StokeFlowDescriptor3D.cu:6-50).
  *
- * Parity status of each function is stated in oracle/README.md and DESIGN.md.
+ * Each function is pinned by identities, closed forms, dense solves or finite
+ * differences in tests/test_oracle.py.
  */
 #ifndef STOKES_ORACLE_H
 #define STOKES_ORACLE_H
@@ -85,6 +86,12 @@ void orc_prolong_add(int dim, int nx, int ny, int nz, const uint8_t *ffixed, int
                      const double *ec, double *xf);
 void orc_coarsen(int dim, int nx, int ny, int nz, const uint8_t *ffixed, const double *fpen,
                  uint8_t *cfixed, double *cpen);
+/* learned part: FP64 3D convolution (channels-last, zero padding) and its adjoints */
+void orc_conv3d(int R, int ntap, int Cin, int Cout, const double *in, const double *W, const double *bias,
+                double *out, int relu);
+void orc_conv3d_bwd_data(int R, int ntap, int Cin, int Cout, const double *gout, const double *W, double *gin);
+void orc_conv3d_bwd_weight(int R, int ntap, int Cin, int Cout, const double *in, const double *gout, double *gW,
+                           double *gb);
 /* FGMRES(m), right-preconditioned by one V-cycle; returns iterations, relres out */
 int orc_fgmres(orc_mg *mg, const double *b, double *x, double rtol, int restart, int maxit,
                double *relres, double *history);

@@ -72,9 +72,16 @@ SIGNATURES = {
     "st_sensitivity": (I, [C, D, D, D, D]),
     "st_filter": (I, [C, D, D, I]),
     "st_oc_update": (I, [C, D, D, F, F, F, F, D, D]),
-    "st_to_step": (I, [C, F, F, F, F, D, D, D, ctypes.POINTER(StStats), ctypes.POINTER(StStats)]),
+    "st_to_step": (I, [C, I, F, F, F, F, D, D, D, ctypes.POINTER(StStats), ctypes.POINTER(StStats)]),
     "st_record_sample": (I, [C]),
+    "st_num_samples": (I, [C]),
     "st_net_train": (I, [C, I, F, D]),
+    "st_net_num_params": (ctypes.c_long, [C]),
+    "st_net_get_params": (I, [C, ctypes.POINTER(ctypes.c_float)]),
+    "st_net_set_params": (I, [C, ctypes.POINTER(ctypes.c_float)]),
+    "st_net_grad": (I, [C, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float), D,
+                        ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]),
+    "st_net_conv_test": (I, [I, I, I, I, I, I] + [ctypes.POINTER(ctypes.c_float)] * 7),
     "st_mg_levels": (I, [C]),
     "st_mg_level_n": (I, [C, I]),
     "st_mg_vcycle": (I, [C, D, D]),
@@ -112,6 +119,13 @@ class StokesError(RuntimeError):
 def dptr(a):
     assert a.dtype == np.float64 and a.flags.c_contiguous
     return a.ctypes.data_as(D)
+
+
+def fptr(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float32 and a.flags.c_contiguous
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
 
 
 def u8ptr(a):

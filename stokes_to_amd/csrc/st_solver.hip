@@ -12,6 +12,7 @@
 //    and orthogonalises on the host, CX/GMRES.cpp:46-95).
 #include "../../include/stokes_to.h"
 #include "st_kernels.h"
+#include "st_net.h"
 
 #include <cmath>
 #include <cstdio>
@@ -62,7 +63,11 @@ struct st_ctx {
     std::string err;
     std::vector<void *> allocs;
     double last_setup_ms = 0.0;
-    void *net = nullptr;  // learned predictor state
+    NetImpl *net = nullptr;     // learned predictor (created on first use)
+    int lc = -1;                // network level (n/4)
+    const double *rho_used = nullptr;  // density the current alpha was computed from
+    double *xdc = nullptr, *xcp = nullptr, *xcn = nullptr;  // level-lc Dirichlet state, previous / new prediction
+    bool have_pred = false, have_lam = false;
     // kernel timing: event pairs around finest-level launches
     bool prof = false;
     std::vector<hipEvent_t> evpool[ST_PROF_NKIND];
@@ -521,6 +526,7 @@ void st_destroy(st_ctx *c)
 {
     if (!c) return;
     if (c->s) hipStreamSynchronize(c->s);
+    if (c->net) net_destroy(c->net);
     for (void *q : c->allocs) hipFree(q);
     if (c->e0) hipEventDestroy(c->e0);
     if (c->e1) hipEventDestroy(c->e1);
@@ -561,6 +567,7 @@ int st_set_penalty(st_ctx *c, const double *pen)
     CKR(stage_in(c, pen, c->ndof));
     launch_alpha_from_pen<double>(c->lv[0], c->abi_a, c->fl[0], c->h64.al[0], c->s);
     CK(hipStreamSynchronize(c->s));
+    c->rho_used = nullptr;
     c->alpha = true;
     c->dirty = true;
     return ST_OK;
@@ -579,6 +586,7 @@ static AlphaParams alpha_params(const st_ctx *c)
 
 static void design_to_alpha(st_ctx *c, const double *rho_dev)
 {
+    c->rho_used = rho_dev;
     launch_design_alpha(c->lv[0], rho_dev, alpha_params(c), c->fl[0], c->h64.al[0], c->s);
     c->alpha = true;
     c->dirty = true;
@@ -692,6 +700,49 @@ static int solve_core(st_ctx *c, const double *b, double *x, const double *xd, i
     return rc;
 }
 
+// ------------------------------------------------------ learned predictor
+
+static int net_ready(st_ctx *c)
+{
+    if (c->dim != 3) return fail(c, ST_EINVAL, "the learned predictor is 3D only");
+    if (c->net) return ST_OK;
+    if (c->L < 3 || c->lv[2].nx < 4 || c->lv[2].nx % 4) return fail(c, ST_EINVAL, "grid too small for the predictor");
+    c->lc = 2;
+    const Lvl &gc = c->lv[c->lc];
+    const long nv = (long)gc.nfield * gc.S;
+    CKR(dalloc(c, &c->xdc, nv));
+    CKR(dalloc(c, &c->xcp, nv));
+    CKR(dalloc(c, &c->xcn, nv));
+    c->net = net_create(gc.nx, 32, 12345u, c->s);
+    if (!c->net) return fail(c, ST_EHIP, "cannot allocate the predictor");
+    return ST_OK;
+}
+
+// fine FP64 padded vector -> level-lc state (two state restrictions)
+static void restrict_state(st_ctx *c, const double *xf, double *xc)
+{
+    launch_state_restrict(c->lv[0], c->lv[1], c->fl[1], xf, c->h64.t[1], c->s);
+    launch_state_restrict(c->lv[1], c->lv[2], c->fl[2], c->h64.t[1], xc, c->s);
+}
+
+// x += P P xc  (prolongation to the finest level, free DOFs only)
+static void prolong_state_add(st_ctx *c, const double *xc, double *x)
+{
+    const Lvl &g1 = c->lv[1];
+    launch_fill<double>((long)g1.nfield * g1.S, c->h64.t[1], 0.0, c->s);
+    launch_prolong<double>(c->lv[1], c->lv[2], c->fl[1], xc, c->h64.t[1], c->s);
+    launch_prolong<double>(c->lv[0], c->lv[1], c->fl[0], c->h64.t[1], x, c->s);
+}
+
+static int predict(st_ctx *c, double *xc)
+{
+    if (!c->rho_used) return fail(c, ST_ESTATE, "the predictor needs a design (st_set_design / st_to_step)");
+    restrict_state(c, c->XD, c->xdc);
+    if (net_predict(c->net, c->lv[c->lc], c->n, c->n / c->lv[c->lc].nx, c->rho_used, c->fl[c->lc], c->xdc, xc) != 0)
+        return fail(c, ST_EINVAL, "predictor/level mismatch");
+    return ST_OK;
+}
+
 int st_solve(st_ctx *c, int init, const double *x0, double *x_out, st_stats *stats)
 {
     if (!c) return ST_EINVAL;
@@ -704,7 +755,34 @@ int st_solve(st_ctx *c, int init, const double *x0, double *x_out, st_stats *sta
     } else if (init == ST_INIT_KEEP) {
         if (!c->have_x) init = ST_INIT_ZERO;
     } else if (init == ST_INIT_LEARNED) {
-        return fail(c, ST_EINVAL, "learned initial guess is not available in this build");
+        CKR(net_ready(c));
+        if (net_num_samples(c->net) < 1) return fail(c, ST_ESTATE, "the predictor is not trained");
+        if (c->dirty) CKR(setup_hierarchy(c));
+        hipEvent_t a, b;
+        CK(hipEventCreate(&a));
+        CK(hipEventCreate(&b));
+        CK(hipEventRecord(a, c->s));
+        launch_axpby<double, double>(c->nvec, 1.0, c->XD, 0.0, c->X, c->s);
+        CKR(predict(c, c->xcn));
+        prolong_state_add(c, c->xcn, c->X);
+        CK(hipEventRecord(b, c->s));
+        CK(hipEventSynchronize(b));
+        float ims = 0.f;
+        hipEventElapsedTime(&ims, a, b);
+        hipEventDestroy(a);
+        hipEventDestroy(b);
+        int rc = solve_core(c, c->B, c->X, c->XD, ST_INIT_KEEP, stats);
+        if (stats) {
+            stats->init_ms = ims;
+            stats->ms += ims;
+        }
+        if (rc == ST_OK || rc == ST_ENOCONV) c->have_x = true;
+        if (x_out && (rc == ST_OK || rc == ST_ENOCONV)) {
+            launch_unpack<double>(c->lv[0], c->X, c->abi_b, c->s);
+            int r2 = stage_out(c, x_out, c->ndof);
+            if (r2 != ST_OK) return r2;
+        }
+        return rc;
     } else if (init != ST_INIT_ZERO) {
         return ST_EINVAL;
     }
@@ -855,21 +933,55 @@ int st_oc_update(st_ctx *c, const double *rho, const double *dcp, double volfrac
     return ST_OK;
 }
 
-int st_to_step(st_ctx *c, double volfrac, double move, double eta, double rho_min, double *J, double *vol,
-               double *change, st_stats *fwd, st_stats *adj)
+int st_to_step(st_ctx *c, int init, double volfrac, double move, double eta, double rho_min, double *J,
+               double *vol, double *change, st_stats *fwd, st_stats *adj)
 {
     if (!c) return ST_EINVAL;
     if (!c->geom || !c->rhs) return fail(c, ST_ESTATE, "geometry and rhs must be set before st_to_step");
+    if (init != ST_INIT_ZERO && init != ST_INIT_KEEP && init != ST_INIT_LEARNED) return ST_EINVAL;
     // filtered physical density -> face alpha
     filter_dev(c, c->rho, c->rhof, 0);
     design_to_alpha(c, c->rhof);
-    int rc = solve_core(c, c->B, c->X, c->XD, ST_INIT_ZERO, fwd);
+    int fmode = (init == ST_INIT_ZERO || !c->have_x) ? ST_INIT_ZERO : ST_INIT_KEEP;
+    float ims = 0.f;
+    if (init == ST_INIT_LEARNED && c->have_x) {
+        CKR(net_ready(c));
+        if (net_num_samples(c->net) > 0) {
+            // x0 = x_prev + P P (net(rho_new) - net(rho_prev)): coarse-grid correction
+            if (c->dirty) CKR(setup_hierarchy(c));
+            hipEvent_t a, b;
+            CK(hipEventCreate(&a));
+            CK(hipEventCreate(&b));
+            CK(hipEventRecord(a, c->s));
+            CKR(predict(c, c->xcn));
+            if (c->have_pred) {
+                const Lvl &gc = c->lv[c->lc];
+                launch_axpby<double, double>((long)gc.nfield * gc.S, -1.0, c->xcp, 1.0, c->xcn, c->s);
+                prolong_state_add(c, c->xcn, c->X);
+                launch_axpby<double, double>((long)gc.nfield * gc.S, 1.0, c->xcp, 1.0, c->xcn, c->s);
+            }
+            launch_axpby<double, double>((long)c->lv[c->lc].nfield * c->lv[c->lc].S, 1.0, c->xcn, 0.0, c->xcp, c->s);
+            c->have_pred = true;
+            CK(hipEventRecord(b, c->s));
+            CK(hipEventSynchronize(b));
+            hipEventElapsedTime(&ims, a, b);
+            hipEventDestroy(a);
+            hipEventDestroy(b);
+        }
+    }
+    int rc = solve_core(c, c->B, c->X, c->XD, fmode, fwd);
+    if (fwd) {
+        fwd->init_ms = ims;
+        fwd->ms += ims;
+    }
     if (rc != ST_OK) return rc;
     c->have_x = true;
     CKR(objective_dev(c, c->X, c->BA, J));
     // adjoint: A lam = G x (A symmetric), homogeneous Dirichlet data
-    rc = solve_core(c, c->BA, c->LAM, c->ZERO, ST_INIT_ZERO, adj);
+    int amode = (init == ST_INIT_ZERO || !c->have_lam) ? ST_INIT_ZERO : ST_INIT_KEEP;
+    rc = solve_core(c, c->BA, c->LAM, c->ZERO, amode, adj);
     if (rc != ST_OK) return rc;
+    c->have_lam = true;
     launch_sensitivity(c->lv[0], c->rhof, alpha_params(c), c->fl[0], c->X, c->LAM, c->dj, c->s);
     filter_dev(c, c->dj, c->dc, 1);
     CKR(oc_dev(c, c->rho, c->dc, volfrac, move, eta, rho_min, c->rn, change, vol));
@@ -878,8 +990,59 @@ int st_to_step(st_ctx *c, double volfrac, double move, double eta, double rho_mi
     return ST_OK;
 }
 
-int st_record_sample(st_ctx *c) { return c ? fail(c, ST_EINVAL, "learned predictor not built") : ST_EINVAL; }
-int st_net_train(st_ctx *c, int, double, double *) { return c ? fail(c, ST_EINVAL, "learned predictor not built") : ST_EINVAL; }
+int st_record_sample(st_ctx *c)
+{
+    if (!c) return ST_EINVAL;
+    CKR(net_ready(c));
+    if (!c->have_x || !c->rho_used) return fail(c, ST_ESTATE, "record a sample after a solve of a design");
+    restrict_state(c, c->XD, c->xdc);
+    restrict_state(c, c->X, c->xcn);
+    if (net_add_sample(c->net, c->lv[c->lc], c->n, c->n / c->lv[c->lc].nx, c->rho_used, c->fl[c->lc], c->xdc,
+                       c->xcn) != 0)
+        return fail(c, ST_EHIP, "recording a sample failed");
+    return ST_OK;
+}
+
+int st_num_samples(const st_ctx *c) { return (c && c->net) ? net_num_samples(c->net) : 0; }
+
+int st_net_train(st_ctx *c, int epochs, double lr, double *loss)
+{
+    if (!c || epochs < 0) return ST_EINVAL;
+    CKR(net_ready(c));
+    if (net_num_samples(c->net) < 1) return fail(c, ST_ESTATE, "no recorded samples");
+    if (net_train(c->net, epochs, (float)lr, loss) != 0) return fail(c, ST_EHIP, "training failed");
+    return ST_OK;
+}
+
+long st_net_num_params(st_ctx *c)
+{
+    if (!c || net_ready(c) != ST_OK) return -1;
+    return net_num_params(c->net);
+}
+int st_net_get_params(st_ctx *c, float *p)
+{
+    if (!c || !p) return ST_EINVAL;
+    CKR(net_ready(c));
+    return net_get_params(c->net, p) == 0 ? ST_OK : ST_EHIP;
+}
+int st_net_set_params(st_ctx *c, const float *p)
+{
+    if (!c || !p) return ST_EINVAL;
+    CKR(net_ready(c));
+    return net_set_params(c->net, p) == 0 ? ST_OK : ST_EHIP;
+}
+int st_net_grad(st_ctx *c, const float *feat, const float *tgt, double *loss, float *grad, float *out)
+{
+    if (!c || !feat || !tgt || !grad) return ST_EINVAL;
+    CKR(net_ready(c));
+    return net_grad_host(c->net, feat, tgt, loss, grad, out) == 0 ? ST_OK : ST_EHIP;
+}
+int st_net_conv_test(int R, int ntap, int cin, int cout, int relu, int mode, const float *in, const float *w,
+                     const float *bias, const float *gout, float *out, float *gw, float *gb)
+{
+    int r = net_conv_test(R, ntap, cin, cout, relu, mode, in, w, bias, gout, out, gw, gb);
+    return r == 0 ? ST_OK : (r == -1 ? ST_EINVAL : ST_EHIP);
+}
 
 // ------------------------------------------------------ MG test surface
 

@@ -11,7 +11,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import (ST_OK, ST_ENOCONV, ST_INIT_ZERO, ST_INIT_KEEP, ST_INIT_HOST, ST_INIT_LEARNED,
-                   StParams, StStats, StokesError, dptr, u8ptr)
+                   StParams, StStats, StokesError, dptr, fptr, u8ptr)
 from .problems import dof_sizes
 
 _INIT = {"zero": ST_INIT_ZERO, "keep": ST_INIT_KEEP, "host": ST_INIT_HOST, "learned": ST_INIT_LEARNED}
@@ -149,13 +149,51 @@ class FluidSteadyState:
                                           float(eta), float(rho_min), dptr(out), ctypes.byref(ch)))
         return out, ch.value
 
-    def to_step(self, volfrac, move=0.2, eta=0.5, rho_min=1e-3):
+    def to_step(self, volfrac, move=0.2, eta=0.5, rho_min=1e-3, init="zero"):
+        """One design iteration on the device (FT/FluidTopoOpt3D.cpp:4-17); init is
+        'zero' (both solves from zero), 'keep' (previous solutions) or 'learned'
+        (previous solution + learned coarse-grid correction)."""
         J, vol, ch = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
         f, a = StStats(), StStats()
-        self._check(self.lib.st_to_step(self.h, float(volfrac), float(move), float(eta), float(rho_min),
+        mode = _INIT[init] if isinstance(init, str) else int(init)
+        self._check(self.lib.st_to_step(self.h, mode, float(volfrac), float(move), float(eta), float(rho_min),
                                         ctypes.byref(J), ctypes.byref(vol), ctypes.byref(ch), ctypes.byref(f),
                                         ctypes.byref(a)))
         return {"J": J.value, "vol": vol.value, "change": ch.value, "fwd": f.as_dict(), "adj": a.as_dict()}
+
+    # ---- learned predictor ----
+    def record_sample(self):
+        self._check(self.lib.st_record_sample(self.h))
+
+    def num_samples(self):
+        return self.lib.st_num_samples(self.h)
+
+    def net_train(self, epochs, lr=1e-3):
+        loss = ctypes.c_double()
+        self._check(self.lib.st_net_train(self.h, int(epochs), float(lr), ctypes.byref(loss)))
+        return loss.value
+
+    def net_params(self):
+        n = self.lib.st_net_num_params(self.h)
+        if n < 0:
+            self._check(-1)
+        p = np.zeros(n, np.float32)
+        self._check(self.lib.st_net_get_params(self.h, fptr(p)))
+        return p
+
+    def net_set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        self._check(self.lib.st_net_set_params(self.h, fptr(p)))
+
+    def net_grad(self, feat, tgt):
+        feat = np.ascontiguousarray(feat, dtype=np.float32)
+        tgt = np.ascontiguousarray(tgt, dtype=np.float32)
+        n = self.lib.st_net_num_params(self.h)
+        g = np.zeros(n, np.float32)
+        out = np.zeros(feat.size // 4 * 16, np.float32)
+        loss = ctypes.c_double()
+        self._check(self.lib.st_net_grad(self.h, fptr(feat), fptr(tgt), ctypes.byref(loss), fptr(g), fptr(out)))
+        return loss.value, g, out
 
     # ---- multigrid internals (component tests) ----
     def mg_levels(self):
@@ -194,3 +232,24 @@ class FluidSteadyState:
         out = np.zeros_like(_f64(xf))
         self._check(self.lib.st_mg_prolong(self.h, l, dptr(_f64(ec)), dptr(_f64(xf)), dptr(out)))
         return out
+
+
+def conv_test(R, ntap, cin, cout, relu, mode, x=None, w=None, b=None, gout=None, lib_path=None):
+    """One network convolution layer on the GPU through st_net_conv_test (host arrays, FP32
+    in/out, BF16 operands and FP32 accumulation inside)."""
+    lib = _lib.load(lib_path)
+    nv = R ** 3
+    f = lambda a, n: np.ascontiguousarray(a if a is not None else np.zeros(n), dtype=np.float32)
+    x, w, b, gout = f(x, nv * cin), f(w, ntap * cin * cout), f(b, cout), f(gout, nv * cout)
+    out = np.zeros(nv * max(cin, cout), np.float32)
+    gw = np.zeros(ntap * cin * cout, np.float32)
+    gb = np.zeros(cout, np.float32)
+    rc = lib.st_net_conv_test(R, ntap, cin, cout, int(relu), int(mode), fptr(x), fptr(w), fptr(b), fptr(gout),
+                              fptr(out), fptr(gw), fptr(gb))
+    if rc != ST_OK:
+        raise StokesError(rc, "st_net_conv_test failed")
+    if mode == 0:
+        return out[: nv * cout]
+    if mode == 1:
+        return out[: nv * cin]
+    return gw, gb
