@@ -147,10 +147,43 @@ __global__ void __launch_bounds__(256) k_march(Lvl g, const uint8_t *__restrict_
     fetch(ic0 + 2);
     __syncthreads();
 
+    // global operands of the node (i, j, k) and of its three high faces, loaded
+    // before the two LDS phases of the iteration so that their latency overlaps them:
+    // pb = b (p, x, y, z rows; then the x/y/z rows of the high faces), px = raw x of
+    // the own faces then of the high faces (the values of fixed faces), pa = alpha
+    C pb[7], px[6], pa[6];
+    auto preload = [&](int i) {
+        const long q = (long)i * sI + (long)j * sJ + k;
+        if (MODE != M_APPLY) {
+            pb[0] = Cvt<TS>::ld(b[q]);
+            pb[1] = Cvt<TS>::ld(b[S + q]);
+            pb[2] = Cvt<TS>::ld(b[2 * S + q]);
+            pb[3] = Cvt<TS>::ld(b[3 * S + q]);
+        }
+        px[0] = Cvt<TS>::ld(x[S + q]);
+        px[1] = Cvt<TS>::ld(x[2 * S + q]);
+        px[2] = Cvt<TS>::ld(x[3 * S + q]);
+        pa[0] = Cvt<TS>::ld(al[q]);
+        pa[1] = Cvt<TS>::ld(al[S + q]);
+        pa[2] = Cvt<TS>::ld(al[2 * S + q]);
+        if (MODE == M_SMOOTH) {  // i < nx, j < ny, k < nz: the high faces exist
+            pb[4] = Cvt<TS>::ld(b[S + q + sI]);
+            pb[5] = Cvt<TS>::ld(b[2 * S + q + sJ]);
+            pb[6] = Cvt<TS>::ld(b[3 * S + q + 1]);
+            px[3] = Cvt<TS>::ld(x[S + q + sI]);
+            px[4] = Cvt<TS>::ld(x[2 * S + q + sJ]);
+            px[5] = Cvt<TS>::ld(x[3 * S + q + 1]);
+            pa[3] = Cvt<TS>::ld(al[q + sI]);
+            pa[4] = Cvt<TS>::ld(al[S + q + sJ]);
+            pa[5] = Cvt<TS>::ld(al[2 * S + q + 1]);
+        }
+    };
+
     double acc = 0.0;
     for (int i = ic0; i < ic1; i++) {
         store(i + 2);
         if (i + 1 < ic1) fetch(i + 3);
+        preload(i);
         __syncthreads();
         edges(i + 1);
         gdiv(i + 1);
@@ -158,32 +191,33 @@ __global__ void __launch_bounds__(256) k_march(Lvl g, const uint8_t *__restrict_
         const int ps = slot3(i), pn = slot3(i + 1), es = slot2(i), en = slot2(i + 1), gs = slot2(i), gn = slot2(i + 1);
         const long q = (long)i * sI + (long)j * sJ + k;
 
-        // operator row values at plane i, LDS position c (node (i, j0+.., k0+..))
-        auto yp = [&](int c, long qq) -> C { return (sF[ps][c] & 1) ? sG[gs][c] : sP[ps][c]; };
-        auto yx = [&](int c, long qq, C gim1, C pim1) -> C {
-            if (!(sF[ps][c] & 2)) return Cvt<TS>::ld(x[S + qq]);
+        // operator row values at plane i, LDS position c; xr / av are the raw x and
+        // the alpha of that face (used for a fixed face / the Brinkman term)
+        auto yp = [&](int c) -> C { return (sF[ps][c] & 1) ? sG[gs][c] : sP[ps][c]; };
+        auto yx = [&](int c, C gim1, C pim1, C xr, C av) -> C {
             C r = gim1 - sG[gs][c] + sE[es][1][c + 1] - sE[es][1][c] + sE[es][2][c] - sE[es][2][c + LK];
-            return r + pim1 - sP[ps][c] + Cvt<TS>::ld(al[qq]) * sW[ps][0][c];
+            r = r + pim1 - sP[ps][c] + av * sW[ps][0][c];
+            return (sF[ps][c] & 2) ? r : xr;
         };
-        auto yy = [&](int c, long qq) -> C {
-            if (!(sF[ps][c] & 4)) return Cvt<TS>::ld(x[2 * S + qq]);
+        auto yy = [&](int c, C xr, C av) -> C {
             C r = sG[gs][c - LK] - sG[gs][c] + sE[es][0][c] - sE[es][0][c + 1] + sE[en][2][c] - sE[es][2][c];
-            return r + sP[ps][c - LK] - sP[ps][c] + Cvt<TS>::ld(al[S + qq]) * sW[ps][1][c];
+            r = r + sP[ps][c - LK] - sP[ps][c] + av * sW[ps][1][c];
+            return (sF[ps][c] & 4) ? r : xr;
         };
-        auto yz = [&](int c, long qq) -> C {
-            if (!(sF[ps][c] & 8)) return Cvt<TS>::ld(x[3 * S + qq]);
+        auto yz = [&](int c, C xr, C av) -> C {
             C r = sG[gs][c - 1] - sG[gs][c] + sE[es][0][c + LK] - sE[es][0][c] + sE[es][1][c] - sE[en][1][c];
-            return r + sP[ps][c - 1] - sP[ps][c] + Cvt<TS>::ld(al[2 * S + qq]) * sW[ps][2][c];
+            r = r + sP[ps][c - 1] - sP[ps][c] + av * sW[ps][2][c];
+            return (sF[ps][c] & 8) ? r : xr;
         };
 
         if (MODE != M_SMOOTH) {
-            auto emit = [&](int c, long qq, C gim1, C pim1) {
-                C v0 = yp(c, qq), v1 = yx(c, qq, gim1, pim1), v2 = yy(c, qq), v3 = yz(c, qq);
+            auto emit = [&](int c, long qq, C gim1, C pim1, const C *bb, const C *xr, const C *av) {
+                C v0 = yp(c), v1 = yx(c, gim1, pim1, xr[0], av[0]), v2 = yy(c, xr[1], av[1]), v3 = yz(c, xr[2], av[2]);
                 if (MODE == M_RESID) {
-                    v0 = Cvt<TS>::ld(b[qq]) - v0;
-                    v1 = Cvt<TS>::ld(b[S + qq]) - v1;
-                    v2 = Cvt<TS>::ld(b[2 * S + qq]) - v2;
-                    v3 = Cvt<TS>::ld(b[3 * S + qq]) - v3;
+                    v0 = bb[0] - v0;
+                    v1 = bb[1] - v1;
+                    v2 = bb[2] - v2;
+                    v3 = bb[3] - v3;
                     acc += (double)v0 * v0 + (double)v1 * v1 + (double)v2 * v2 + (double)v3 * v3;
                 }
                 y[qq] = Cvt<TS>::st(v0);
@@ -191,12 +225,23 @@ __global__ void __launch_bounds__(256) k_march(Lvl g, const uint8_t *__restrict_
                 y[2 * S + qq] = Cvt<TS>::st(v2);
                 y[3 * S + qq] = Cvt<TS>::st(v3);
             };
-            emit(cown, q, gm1, pm1);
+            emit(cown, q, gm1, pm1, pb, px, pa);
             // the padded node column k = nz and row j = ny belong to the last tiles
             const bool lastk = (k == g.nz - 1), lastj = (j == g.ny - 1);
-            if (lastk) emit(cown + 1, q + 1, C(0), C(0));
-            if (lastj) emit(cown + LK, q + sJ, C(0), C(0));
-            if (lastk && lastj) emit(cown + LK + 1, q + sJ + 1, C(0), C(0));
+            auto emit_edge = [&](int c, long qq) {
+                C bb[4], xr[3], av[3];
+#pragma unroll
+                for (int f = 0; f < 4; f++) bb[f] = (MODE == M_RESID) ? Cvt<TS>::ld(b[(long)f * S + qq]) : C(0);
+#pragma unroll
+                for (int f = 0; f < 3; f++) {
+                    xr[f] = Cvt<TS>::ld(x[(long)(f + 1) * S + qq]);
+                    av[f] = Cvt<TS>::ld(al[(long)f * S + qq]);
+                }
+                emit(c, qq, C(0), C(0), bb, xr, av);
+            };
+            if (lastk) emit_edge(cown + 1, q + 1);
+            if (lastj) emit_edge(cown + LK, q + sJ);
+            if (lastk && lastj) emit_edge(cown + LK + 1, q + sJ + 1);
         } else {
             // ---- one colour of the box-Vanka smoother for the cells of plane i
             const int c = cown;
@@ -204,35 +249,31 @@ __global__ void __launch_bounds__(256) k_march(Lvl g, const uint8_t *__restrict_
             const int fb = sF[ps][c];
             if (!mine) {
                 y[q] = Cvt<TS>::st(sP[ps][c]);
-                if (i == 0) y[S + q] = x[S + q];
-                if (i == g.nx - 1) y[S + q + sI] = x[S + q + sI];
-                if (j == 0) y[2 * S + q] = x[2 * S + q];
-                if (j == g.ny - 1) y[2 * S + q + sJ] = x[2 * S + q + sJ];
-                if (k == 0) y[3 * S + q] = x[3 * S + q];
-                if (k == g.nz - 1) y[3 * S + q + 1] = x[3 * S + q + 1];
+                if (i == 0) y[S + q] = Cvt<TS>::st(px[0]);
+                if (i == g.nx - 1) y[S + q + sI] = Cvt<TS>::st(px[3]);
+                if (j == 0) y[2 * S + q] = Cvt<TS>::st(px[1]);
+                if (j == g.ny - 1) y[2 * S + q + sJ] = Cvt<TS>::st(px[4]);
+                if (k == 0) y[3 * S + q] = Cvt<TS>::st(px[2]);
+                if (k == g.nz - 1) y[3 * S + q + 1] = Cvt<TS>::st(px[5]);
             } else {
                 const C mc = (fb & 1) ? C(1) : C(0);
                 const long qh[3] = {q + sI, q + sJ, q + 1};
                 // residuals at the 7 DOFs
-                const C rp = Cvt<TS>::ld(b[q]) - yp(c, q);
+                const C rp = pb[0] - yp(c);
                 C r0[3], r1[3], u0[3], u1[3], m0[3], m1[3], a0[3], a1[3];
                 // low faces (node i,j,k)
-                r0[0] = Cvt<TS>::ld(b[S + q]) - yx(c, q, gm1, pm1);
-                r0[1] = Cvt<TS>::ld(b[2 * S + q]) - yy(c, q);
-                r0[2] = Cvt<TS>::ld(b[3 * S + q]) - yz(c, q);
+                r0[0] = pb[1] - yx(c, gm1, pm1, px[0], pa[0]);
+                r0[1] = pb[2] - yy(c, px[1], pa[1]);
+                r0[2] = pb[3] - yz(c, px[2], pa[2]);
                 // high x face: node (i+1, j, k), from plane i+1 (slots pn, en, gn)
                 {
                     const int f1 = sF[pn][c];
-                    C v;
-                    if (!(f1 & 2)) v = Cvt<TS>::ld(x[S + qh[0]]);
-                    else {
-                        v = sG[gs][c] - sG[gn][c] + sE[en][1][c + 1] - sE[en][1][c] + sE[en][2][c] - sE[en][2][c + LK];
-                        v += sP[ps][c] - sP[pn][c] + Cvt<TS>::ld(al[qh[0]]) * sW[pn][0][c];
-                    }
-                    r1[0] = Cvt<TS>::ld(b[S + qh[0]]) - v;
+                    C v = sG[gs][c] - sG[gn][c] + sE[en][1][c + 1] - sE[en][1][c] + sE[en][2][c] - sE[en][2][c + LK];
+                    v += sP[ps][c] - sP[pn][c] + pa[3] * sW[pn][0][c];
+                    r1[0] = pb[4] - ((f1 & 2) ? v : px[3]);
                 }
-                r1[1] = Cvt<TS>::ld(b[2 * S + qh[1]]) - yy(c + LK, qh[1]);
-                r1[2] = Cvt<TS>::ld(b[3 * S + qh[2]]) - yz(c + 1, qh[2]);
+                r1[1] = pb[5] - yy(c + LK, px[4], pa[4]);
+                r1[2] = pb[6] - yz(c + 1, px[5], pa[5]);
                 // masks, raw values, diagonals
                 const int fl0[3] = {fb, fb, fb};
                 const int fl1[3] = {sF[pn][c], sF[ps][c + LK], sF[ps][c + 1]};
@@ -243,10 +284,10 @@ __global__ void __launch_bounds__(256) k_march(Lvl g, const uint8_t *__restrict_
                 for (int d = 0; d < 3; d++) {
                     m0[d] = ((fl0[d] >> (1 + d)) & 1) ? C(1) : C(0);
                     m1[d] = ((fl1[d] >> (1 + d)) & 1) ? C(1) : C(0);
-                    u0[d] = (m0[d] != C(0)) ? sW[ps][d][c] : Cvt<TS>::ld(x[(long)(1 + d) * S + q]);
+                    u0[d] = (m0[d] != C(0)) ? sW[ps][d][c] : px[d];
                     const int slot1 = (d == 0) ? pn : ps;
-                    u1[d] = (m1[d] != C(0)) ? sW[slot1][d][cw1[d]] : Cvt<TS>::ld(x[(long)(1 + d) * S + qh[d]]);
-                    const C al0 = Cvt<TS>::ld(al[(long)d * S + q]), al1 = Cvt<TS>::ld(al[(long)d * S + qh[d]]);
+                    u1[d] = (m1[d] != C(0)) ? sW[slot1][d][cw1[d]] : px[3 + d];
+                    const C al0 = pa[d], al1 = pa[3 + d];
                     a0[d] = m0[d] * (clo[d] + mc + C(4)) + al0 + (C(1) - m0[d]);
                     a1[d] = m1[d] * (mc + chi[d] + C(4)) + al1 + (C(1) - m1[d]);
                 }

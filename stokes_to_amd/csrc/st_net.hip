@@ -675,7 +675,7 @@ NetImpl *net_create(int R, int max_samples, unsigned seed, hipStream_t s)
          n->alloc(&n->gP2, v2 * 32) && n->alloc(&n->gE2a, v1 * 32) && n->alloc(&n->gP1, v1 * 16) &&
          n->alloc(&n->gE1a, v0 * 16);
     ok = ok && n->alloc(&n->part, maxparts * n->nsplit) && n->alloc(&n->lpart, nblk(v0) + 1) &&
-         n->alloc(&n->feat, (long)max_samples * v0 * 4 + 1) && n->alloc(&n->tgt_raw, (long)max_samples * v0 * 4 + 1) &&
+         n->alloc(&n->feat, (long)(max_samples + 1) * v0 * 4) && n->alloc(&n->tgt_raw, (long)max_samples * v0 * 4 + 1) &&
          n->alloc(&n->tgt, v0 * 4) && n->alloc(&n->scale, NET_NOUT);
     if (!ok) {
         net_destroy(n);
@@ -828,7 +828,7 @@ int net_predict(NetImpl *n, const Lvl &gc, int nf, int ratio, const double *rho,
 
 int net_grad_host(NetImpl *n, const float *feat, const float *tgt, double *loss, float *grad, float *out)
 {
-    float *dfeat = n->feat, *dt = n->tgt;
+    float *dfeat = n->feat + (long)n->maxs * n->nv0 * 4, *dt = n->tgt;  // staging slot after the samples
     hipMemcpyAsync(dfeat, feat, sizeof(float) * n->nv0 * 4, hipMemcpyHostToDevice, n->s);
     hipMemcpyAsync(dt, tgt, sizeof(float) * n->nv0 * 4, hipMemcpyHostToDevice, n->s);
     hipLaunchKernelGGL(k_feat_pad, dim3(nblk(n->nv0 * 16)), dim3(256), 0, n->s, n->nv0, dfeat, n->X0);

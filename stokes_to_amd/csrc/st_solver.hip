@@ -743,6 +743,42 @@ static int predict(st_ctx *c, double *xc)
     return ST_OK;
 }
 
+// Learned initial guess in X, returns its device time in ms.  After a learned solve
+// of a previous design: X = X_prev + P P (net(rho) - net(rho_prev)), the previous
+// solution corrected by the predicted change of the coarse state; after another
+// solve: X = X_prev; with no solution yet: X = x_D + P P net(rho).
+static int learned_guess(st_ctx *c, float *ms)
+{
+    CKR(net_ready(c));
+    if (net_num_samples(c->net) < 1) return fail(c, ST_ESTATE, "the predictor is not trained");
+    if (c->dirty) CKR(setup_hierarchy(c));
+    const Lvl &gc = c->lv[c->lc];
+    const long nvc = (long)gc.nfield * gc.S;
+    hipEvent_t a, b;
+    CK(hipEventCreate(&a));
+    CK(hipEventCreate(&b));
+    CK(hipEventRecord(a, c->s));
+    CKR(predict(c, c->xcn));
+    if (c->have_x && c->have_pred) {
+        launch_axpby<double, double>(nvc, -1.0, c->xcp, 1.0, c->xcn, c->s);  // xcn = new - prev
+        prolong_state_add(c, c->xcn, c->X);
+        launch_axpby<double, double>(nvc, 1.0, c->xcp, 1.0, c->xcn, c->s);   // restore new
+    } else if (!c->have_x) {
+        launch_axpby<double, double>(c->nvec, 1.0, c->XD, 0.0, c->X, c->s);
+        prolong_state_add(c, c->xcn, c->X);
+    }
+    launch_axpby<double, double>(nvc, 1.0, c->xcn, 0.0, c->xcp, c->s);
+    c->have_pred = true;
+    CK(hipEventRecord(b, c->s));
+    CK(hipEventSynchronize(b));
+    float ims = 0.f;
+    hipEventElapsedTime(&ims, a, b);
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    if (ms) *ms = ims;
+    return ST_OK;
+}
+
 int st_solve(st_ctx *c, int init, const double *x0, double *x_out, st_stats *stats)
 {
     if (!c) return ST_EINVAL;
@@ -755,22 +791,8 @@ int st_solve(st_ctx *c, int init, const double *x0, double *x_out, st_stats *sta
     } else if (init == ST_INIT_KEEP) {
         if (!c->have_x) init = ST_INIT_ZERO;
     } else if (init == ST_INIT_LEARNED) {
-        CKR(net_ready(c));
-        if (net_num_samples(c->net) < 1) return fail(c, ST_ESTATE, "the predictor is not trained");
-        if (c->dirty) CKR(setup_hierarchy(c));
-        hipEvent_t a, b;
-        CK(hipEventCreate(&a));
-        CK(hipEventCreate(&b));
-        CK(hipEventRecord(a, c->s));
-        launch_axpby<double, double>(c->nvec, 1.0, c->XD, 0.0, c->X, c->s);
-        CKR(predict(c, c->xcn));
-        prolong_state_add(c, c->xcn, c->X);
-        CK(hipEventRecord(b, c->s));
-        CK(hipEventSynchronize(b));
         float ims = 0.f;
-        hipEventElapsedTime(&ims, a, b);
-        hipEventDestroy(a);
-        hipEventDestroy(b);
+        CKR(learned_guess(c, &ims));
         int rc = solve_core(c, c->B, c->X, c->XD, ST_INIT_KEEP, stats);
         if (stats) {
             stats->init_ms = ims;
@@ -786,6 +808,7 @@ int st_solve(st_ctx *c, int init, const double *x0, double *x_out, st_stats *sta
     } else if (init != ST_INIT_ZERO) {
         return ST_EINVAL;
     }
+    c->have_pred = false;  // the stored prediction no longer matches the solution
     int rc = solve_core(c, c->B, c->X, c->XD, init == ST_INIT_HOST ? ST_INIT_KEEP : init, stats);
     if (rc == ST_OK || rc == ST_ENOCONV) c->have_x = true;
     if (x_out && (rc == ST_OK || rc == ST_ENOCONV)) {
@@ -944,31 +967,10 @@ int st_to_step(st_ctx *c, int init, double volfrac, double move, double eta, dou
     design_to_alpha(c, c->rhof);
     int fmode = (init == ST_INIT_ZERO || !c->have_x) ? ST_INIT_ZERO : ST_INIT_KEEP;
     float ims = 0.f;
-    if (init == ST_INIT_LEARNED && c->have_x) {
-        CKR(net_ready(c));
-        if (net_num_samples(c->net) > 0) {
-            // x0 = x_prev + P P (net(rho_new) - net(rho_prev)): coarse-grid correction
-            if (c->dirty) CKR(setup_hierarchy(c));
-            hipEvent_t a, b;
-            CK(hipEventCreate(&a));
-            CK(hipEventCreate(&b));
-            CK(hipEventRecord(a, c->s));
-            CKR(predict(c, c->xcn));
-            if (c->have_pred) {
-                const Lvl &gc = c->lv[c->lc];
-                launch_axpby<double, double>((long)gc.nfield * gc.S, -1.0, c->xcp, 1.0, c->xcn, c->s);
-                prolong_state_add(c, c->xcn, c->X);
-                launch_axpby<double, double>((long)gc.nfield * gc.S, 1.0, c->xcp, 1.0, c->xcn, c->s);
-            }
-            launch_axpby<double, double>((long)c->lv[c->lc].nfield * c->lv[c->lc].S, 1.0, c->xcn, 0.0, c->xcp, c->s);
-            c->have_pred = true;
-            CK(hipEventRecord(b, c->s));
-            CK(hipEventSynchronize(b));
-            hipEventElapsedTime(&ims, a, b);
-            hipEventDestroy(a);
-            hipEventDestroy(b);
-        }
-    }
+    if (init == ST_INIT_LEARNED && c->have_x && c->net && net_num_samples(c->net) > 0)
+        CKR(learned_guess(c, &ims));
+    else
+        c->have_pred = false;
     int rc = solve_core(c, c->B, c->X, c->XD, fmode, fwd);
     if (fwd) {
         fwd->init_ms = ims;

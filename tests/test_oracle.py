@@ -233,3 +233,93 @@ def test_grf_density_seeded_mean():
     b = grf_density(3, 16, mean=0.3, sigma=2.0, seed=12345)
     assert np.array_equal(a, b)
     assert abs(a.mean() - 0.3) < 1e-6 and a.min() >= 1e-3 and a.max() <= 1.0
+
+
+# ---------------------------------------------------------------- learned part
+
+@pytest.mark.parametrize("R,ntap,cin,cout", [(5, 27, 3, 4), (4, 1, 5, 2)])
+def test_conv3d_matches_library_correlation(orc, R, ntap, cin, cout):
+    """Forward convolution against scipy.ndimage.correlate (zero padding); the
+    input-gradient is its adjoint; the weight-gradient is the derivative of the
+    (linear-in-W) pairing <conv(x; W), g>; the bias gradient is the sum of g."""
+    from scipy.ndimage import correlate
+    rng = np.random.default_rng(R + cin)
+    nv = R ** 3
+    x = rng.standard_normal(nv * cin)
+    w = rng.standard_normal(ntap * cin * cout)
+    b = rng.standard_normal(cout)
+    y = orc.conv3d(R, ntap, cin, cout, x, w, b).reshape(R, R, R, cout)
+    xs = x.reshape(R, R, R, cin)
+    k = 3 if ntap == 27 else 1
+    W = w.reshape(k, k, k, cin, cout)   # tap t = (a+1)*9 + (b+1)*3 + (c+1) <-> offset (a, b, c)
+    ref = np.zeros((R, R, R, cout))
+    for co in range(cout):
+        ref[..., co] = b[co] + sum(correlate(xs[..., ci], W[..., ci, co], mode="constant", cval=0.0)
+                                   for ci in range(cin))
+    assert np.abs(y - ref).max() < 1e-12 * np.abs(ref).max()
+    yr = orc.conv3d(R, ntap, cin, cout, x, w, b, relu=True).reshape(y.shape)
+    assert np.array_equal(yr, np.maximum(y, 0.0))
+    g = rng.standard_normal(nv * cout)
+    y0 = orc.conv3d(R, ntap, cin, cout, x, w, None)
+    gi = orc.conv3d_bwd_data(R, ntap, cin, cout, g, w)
+    assert abs(y0 @ g - x @ gi) < 1e-12 * np.abs(g).sum() * np.abs(y0).max()
+    gw, gb = orc.conv3d_bwd_weight(R, ntap, cin, cout, x, g)
+    w2 = rng.standard_normal(w.size)
+    assert abs(orc.conv3d(R, ntap, cin, cout, x, w2, None) @ g - w2 @ gw) < 1e-11 * np.abs(gw).sum()
+    assert np.allclose(gb, g.reshape(nv, cout).sum(axis=0), rtol=1e-13, atol=1e-12)
+
+
+def test_network_pool_and_upsample_adjoints():
+    import oracle.net as N
+    rng = np.random.default_rng(4)
+    R, C = 4, 3
+    x = rng.standard_normal(R ** 3 * C)
+    g = rng.standard_normal((R // 2) ** 3 * C)
+    assert abs(N.pool(x, R, C) @ g - x @ N.pool_bwd(g, R, C)) < 1e-12
+    assert abs(N.up(g, R // 2, C) @ x - g @ N.up_bwd(x, R, C)) < 1e-12
+    # pooling a constant field keeps it; upsampling then pooling is the identity
+    assert np.allclose(N.pool(np.ones(R ** 3 * C), R, C), 1.0)
+    assert np.allclose(N.pool(N.up(g, R // 2, C), R, C), g)
+
+
+def test_network_gradient_matches_finite_differences():
+    """Loss gradient of the oracle network (plain FP64) by central differences,
+    coordinate-wise on a sample of parameters and along a random direction."""
+    import oracle.net as N
+    R = 4
+    rng = np.random.default_rng(0)
+    npar = sum(nt * ci * co + co for ci, co, nt, _, _ in N.SPEC)
+    p = rng.standard_normal(npar) * 0.15
+    feat = rng.standard_normal(R ** 3 * 4)
+    tgt = rng.standard_normal(R ** 3 * 4)
+    L, g, _ = N.loss_and_grad(p, feat, tgt, R)
+    h = 1e-5
+    f = lambda q: N.loss_and_grad(q, feat, tgt, R)[0]
+    # one coordinate from every layer's weights and biases
+    idx, o = [], 0
+    for ci, co, nt, _, _ in N.SPEC:
+        idx += [o + int(rng.integers(nt * ci * co)), o + nt * ci * co + int(rng.integers(co))]
+        o += nt * ci * co + co
+    for i in idx:
+        e = np.zeros(npar)
+        e[i] = h
+        fd = (f(p + e) - f(p - e)) / (2 * h)
+        assert abs(fd - g[i]) <= 1e-6 * max(abs(g[i]), 1e-3 * np.abs(g).max()), i
+    d = rng.standard_normal(npar)
+    d /= np.linalg.norm(d)
+    fd = (f(p + h * d) - f(p - h * d)) / (2 * h)
+    assert abs(fd - g @ d) <= 1e-6 * abs(g @ d)
+
+
+def test_bf16_rounding_known_values():
+    import oracle.net as N
+    u = 2.0 ** -8
+    v = np.array([1.0, 1 + u, 1 + 3 * u, 1 + u + u / 16, -2.5, 3.0e38, 1e-40])
+    r = N.bf16_round(v)
+    # ties go to the even mantissa; above the tie rounds up; sign and range kept
+    assert r[0] == 1.0 and r[1] == 1.0 and r[2] == 1 + 4 * u and r[3] == 1 + 2 * u
+    assert r[4] == -2.5 and abs(r[5] / 3.0e38 - 1) < 2 ** -8
+    # BF16 keeps FP32's exponent range: 1e-40 is an FP32 subnormal that BF16 keeps approximately
+    assert abs(r[6] / 1e-40 - 1) < 2 ** -2
+    # idempotent
+    assert np.array_equal(N.bf16_round(r), r)
