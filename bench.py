@@ -1,15 +1,32 @@
 #!/usr/bin/env python3
 """Benchmark: ms per Stokes solve to a 1e-8 true FP64 relative residual at 3D 256^3,
-plain FP64 GMG-Krylov vs the mixed-precision (and learned) solve, on one MI355X.
+plain FP64 GMG-Krylov vs the learned + mixed-precision solve, on one MI355X.
 
-A "step" is one solve of the 3D 256^3 channel problem (parabolic inflow on a
-centred 1/4 patch of x=0, zero-traction outflow patch on x=1, no-slip elsewhere,
-Brinkman alpha = 1e4*0.1(1-r)/(0.1+r)) on a seeded Gaussian-random-field density
-of mean 0.2, including the design->alpha map and the multigrid hierarchy setup
-that every design iteration repeats.  Inputs are resident in HBM before timing.
+Default workload ("to"): a density-filtered topology-optimization run on the 3D
+256^3 channel problem (parabolic inflow on a centred 1/4 patch of x=0,
+zero-traction outflow patch on x=1, no-slip elsewhere; Brinkman
+alpha = 1e4*0.1(1-r)/(0.1+r); volume fraction 0.2, initial density 0.2, filter
+radius 1.5, OC move 0.2).  The design iterations are driven by the learned+MP
+context (FP32 or BF16 multigrid inside FP64 FGMRES, forward solve from the learned
+initial guess, adjoint from the previous adjoint).  At every iteration the same
+filtered design is also solved by
+  * the plain path: FP64 multigrid, zero initial guess (the reference's GMRES
+    starts from zero), and
+  * the warm-start path: the same mixed precision, previous solution as initial
+    guess, no network (what the method gains without the network).
+The first --warmup iterations are the online training window: after each, the
+solver pair (design, coarse state) is recorded and the network trained for
+--epochs epochs (training time reported separately).  The next --steps
+iterations are timed with the network frozen.  A "step" is one forward solve;
+the time of a solve is HIP events around hierarchy setup, initial guess
+(network inference and its residual check) and FGMRES on the solver stream,
+with every input resident in HBM.
 
-Prints one JSON line (see the keys below).  --gpus N with N > 1 is rejected:
-this workload runs on one GPU.
+Workload "grf": a single solve of a seeded Gaussian-random-field density (mean
+0.2) repeated --steps times, FP64 vs the mixed-precision path (kernel profile).
+
+Prints one JSON line.  --gpus N with N > 1 is rejected: this workload runs on
+one GPU.
 """
 import argparse
 import ctypes
@@ -24,17 +41,22 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+METRIC = "ms per Stokes solve to 1e-8 rel residual at 3D 256³: plain FP64 MG vs learned+MP"
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10, help="timed design iterations (solves)")
+    ap.add_argument("--warmup", type=int, default=15, help="untimed iterations (online training window)")
+    ap.add_argument("--workload", default="to", choices=["to", "grf"])
     ap.add_argument("--n", type=int, default=256)
-    ap.add_argument("--mean", type=float, default=0.2)
+    ap.add_argument("--volfrac", type=float, default=0.2)
+    ap.add_argument("--mean", type=float, default=0.2, help="GRF mean (grf workload)")
     ap.add_argument("--rtol", type=float, default=1e-8)
     ap.add_argument("--mp", default="fp32", choices=["fp32", "bf16"])
+    ap.add_argument("--epochs", type=int, default=3)
+    ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=1)
     ap.add_argument("--quiet", action="store_true")
@@ -48,54 +70,123 @@ def log(args, *a):
         print(*a, file=sys.stderr, flush=True)
 
 
-def run_path(args, prob, rho_dev, precision, profile):
-    import torch
-    from stokes_to_amd import _lib
+def new_ctx(args, prob, precision):
     from stokes_to_amd.steady import FluidSteadyState
-
     s = FluidSteadyState(3, args.n, precision=precision, rtol=args.rtol)
     s.init_boundary(prob.fixed)
     s.update_b(dirichlet=prob.dirichlet)
-    lib = s.lib
-    ptr = ctypes.c_void_p(rho_dev.data_ptr())
+    return s
 
-    def step():
-        rc = lib.st_set_design_device(s.h, ctypes.cast(ptr, ctypes.POINTER(ctypes.c_double)))
-        s._check(rc)
-        _, st = s.solve(init="zero", fetch=False)
-        return st
 
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    if profile:
-        lib.st_profile(s.h, 1)
-    stats = []
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        stats.append(step())
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
+def read_profile(s):
     prof = {}
-    if profile:
-        lib.st_profile(s.h, 0)
-        names = {0: "smooth", 1: "apply", 2: "residual"}
-        for k, nm in names.items():
-            tot, cnt, by = ctypes.c_double(), ctypes.c_long(), ctypes.c_double()
-            lib.st_profile_read(s.h, k, ctypes.byref(tot), ctypes.byref(cnt), ctypes.byref(by))
-            prof[nm] = {"total_ms": tot.value, "launches": cnt.value, "bytes": by.value}
-    s.close()
-    wall_ms = (t1 - t0) * 1e3 / args.steps
+    names = {0: "smooth", 1: "apply", 2: "residual"}
+    for k, nm in names.items():
+        tot, cnt, by = ctypes.c_double(), ctypes.c_long(), ctypes.c_double()
+        s.lib.st_profile_read(s.h, k, ctypes.byref(tot), ctypes.byref(cnt), ctypes.byref(by))
+        prof[nm] = {"total_ms": tot.value, "launches": cnt.value, "bytes": by.value}
+    return prof
+
+
+def summary(stats):
+    ms = [st["ms"] for st in stats]
     return {
-        "wall_ms": wall_ms,
-        "event_ms": float(np.median([st["ms"] for st in stats])),
-        "setup_ms": float(np.median([st["setup_ms"] for st in stats])),
-        "iters": int(np.median([st["iters"] for st in stats])),
-        "relres": float(max(st["relres"] for st in stats)),
-        "converged": all(st["converged"] for st in stats),
-        "prof": prof,
+        "median_ms": float(np.median(ms)), "mean_ms": float(np.mean(ms)),
+        "iters": [int(st["iters"]) for st in stats],
+        "relres_max": float(max(st["relres"] for st in stats)),
+        "converged": all(bool(st["converged"]) for st in stats),
+        "setup_ms_median": float(np.median([st["setup_ms"] for st in stats])),
     }
+
+
+def roofline_of(prof):
+    pr = prof["smooth"]
+    avg_ms = pr["total_ms"] / max(pr["launches"], 1)
+    achieved = pr["bytes"] / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    return {
+        "bound": "hbm", "kernel": "k_march<double,SMOOTH> (256^3 finest-level colour pass, plain FP64 path)",
+        "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+        "avg_launch_ms": round(avg_ms, 4), "launches": pr["launches"],
+        "algorithmic_bytes_per_launch": pr["bytes"],
+    }
+
+
+def run_to(args, prob):
+    """The topology-optimization workload; returns (plain, learned, keep, extra)."""
+    import torch
+    L = new_ctx(args, prob, args.mp)
+    Z = new_ctx(args, prob, "fp64")
+    K = new_ctx(args, prob, args.mp)
+    rho0 = np.full(args.n ** 3, args.volfrac)
+    L.set_design(rho0)
+    plain, learned, keep, train_ms, inf_ms, used = [], [], [], [], [], []
+    total = args.warmup + args.steps
+    for it in range(total):
+        timed = it >= args.warmup
+        if it == args.warmup:
+            torch.cuda.synchronize()
+            Z.lib.st_profile(Z.h, 1)
+        rho = L.get_design()
+        rhof = L.filter(rho)
+        Z.set_design(rhof)
+        K.set_design(rhof)
+        _, sz = Z.solve(init="zero", fetch=False)
+        _, sk = K.solve(init="keep", fetch=False)
+        r = L.to_step(args.volfrac, init="learned")
+        sl = r["fwd"]
+        if timed:
+            plain.append(sz)
+            keep.append(sk)
+            learned.append(sl)
+            inf_ms.append(sl["init_ms"])
+            used.append(int(sl["learned_used"]))
+        else:
+            t0 = time.perf_counter()
+            L.record_sample()
+            L.net_train(args.epochs, args.lr)
+            torch.cuda.synchronize()
+            train_ms.append((time.perf_counter() - t0) * 1e3)
+        log(args, f"[bench] it {it:2d} {'timed' if timed else 'train'} change {r['change']:.3f} J {r['J']:.6g} | "
+                  f"plain {sz['iters']} it {sz['ms']:.1f} ms | keep {sk['iters']} it {sk['ms']:.1f} ms "
+                  f"r0 {sk['relres0']:.1e} | learned {sl['iters']} it {sl['ms']:.1f} ms r0 {sl['relres0']:.1e} "
+                  f"used {sl['learned_used']}")
+    torch.cuda.synchronize()
+    Z.lib.st_profile(Z.h, 0)
+    prof = read_profile(Z)
+    last_rhof = L.filter(L.get_design())
+    extra = {"train_ms_per_iter": train_ms, "train_ms_total": float(sum(train_ms)),
+             "inference_ms_median": float(np.median(inf_ms)) if inf_ms else 0.0,
+             "learned_taken": int(sum(used)), "prof": prof, "last_design": last_rhof,
+             "final_J": r["J"], "final_vol": r["vol"]}
+    for s in (L, Z, K):
+        s.close()
+    return summary(plain), summary(learned), summary(keep), extra
+
+
+def run_grf(args, prob):
+    import torch
+    from stokes_to_amd.problems import grf_density
+    rho = grf_density(3, args.n, mean=args.mean, sigma=8.0, seed=12345)
+    rho_dev = torch.from_numpy(rho).to("cuda")
+    out = {}
+    for prec in ("fp64", args.mp):
+        s = new_ctx(args, prob, prec)
+        ptr = ctypes.cast(ctypes.c_void_p(rho_dev.data_ptr()), ctypes.POINTER(ctypes.c_double))
+        stats = []
+        for i in range(1 + args.steps):
+            if i == 1:
+                torch.cuda.synchronize()
+                s.lib.st_profile(s.h, 1)
+            s._check(s.lib.st_set_design_device(s.h, ptr))
+            _, st = s.solve(init="zero", fetch=False)
+            if i >= 1:
+                stats.append(st)
+        torch.cuda.synchronize()
+        s.lib.st_profile(s.h, 0)
+        out[prec] = (summary(stats), read_profile(s))
+        s.close()
+    return out, rho
 
 
 def cpu_baseline(args, prob, rho, gpu_iters):
@@ -122,9 +213,9 @@ def cpu_baseline(args, prob, rho, gpu_iters):
     est = (t1 - t0) + per_it * gpu_iters
     return {
         "value": est * 1e3, "unit": "ms/solve", "cores": threads, "kind": "port",
-        "sample": f"oracle FGMRES+V(2,2) FP64 on the same {args.n}^3 problem: hierarchy setup "
-                  f"({t1 - t0:.2f} s) + {it} iteration(s) ({per_it:.2f} s each), scaled to the "
-                  f"{gpu_iters} iterations of the FP64 GPU solve",
+        "sample": f"oracle FGMRES+V(2,2) FP64 on the same {args.n}^3 system (the plain path's last design): "
+                  f"hierarchy setup ({t1 - t0:.2f} s) + {it} iteration(s) ({per_it:.2f} s each), scaled to "
+                  f"the {gpu_iters} iterations of the FP64 GPU solve",
     }
 
 
@@ -134,66 +225,63 @@ def main():
         print(f"bench.py: this workload runs on one GPU; --gpus {args.gpus} is not supported", file=sys.stderr)
         return 2
     import torch
-    from stokes_to_amd.problems import channel_problem, grf_density
+    from stokes_to_amd.problems import channel_problem
 
-    t = time.perf_counter()
     prob = channel_problem(3, args.n)
-    rho = grf_density(3, args.n, mean=args.mean, sigma=8.0, seed=12345)
-    log(args, f"[bench] problem {args.n}^3 ready in {time.perf_counter() - t:.1f}s")
-    rho_dev = torch.from_numpy(rho).to("cuda")
-    torch.cuda.synchronize()
+    torch.zeros(1, device="cuda")
     if args.copy_probe_mb > 0:
         # a plain copy of known size: its FETCH_SIZE / WRITE_SIZE counters calibrate
         # the counters read for the solver kernels in the same profiled process
         src = torch.ones(args.copy_probe_mb * (1 << 20) // 8, dtype=torch.float64, device="cuda")
         dst = torch.empty_like(src)
         dst.copy_(src)
+        float(src.sum())          # a pure read of the same size
         torch.cuda.synchronize()
         del src, dst
 
-    plain = run_path(args, prob, rho_dev, "fp64", profile=True)
-    log(args, f"[bench] fp64: {plain['event_ms']:.2f} ms/solve, {plain['iters']} it, relres {plain['relres']:.2e}")
-    mp = run_path(args, prob, rho_dev, args.mp, profile=True)
-    log(args, f"[bench] {args.mp}: {mp['event_ms']:.2f} ms/solve, {mp['iters']} it, relres {mp['relres']:.2e}")
-
-    head = mp if (mp["converged"] and mp["wall_ms"] < plain["wall_ms"]) else plain
-    pr = plain["prof"]["smooth"]
-    avg_ms = pr["total_ms"] / max(pr["launches"], 1)
-    achieved = pr["bytes"] / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
-    out = {
-        "metric": "ms per Stokes solve to 1e-8 rel residual at 3D 256³: plain FP64 MG vs learned+MP",
-        "value": round(head["wall_ms"], 3),
-        "unit": "ms/solve",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(head["wall_ms"], 3),
-        "higher_is_better": False,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64" if head is plain else f"f64+{args.mp}",
-        "data": "synthetic",
-        "config": {
-            "workload": f"3D {args.n}^3 single Stokes/Brinkman solve (channel BCs, GRF density mean {args.mean}, "
-                        f"seed 12345), rtol {args.rtol} true FP64 residual",
-            "n": args.n, "dofs": int(prob.fixed.size), "rtol": args.rtol,
-            "headline_path": "fp64" if head is plain else args.mp,
-        },
-        "plain_fp64": {k: plain[k] for k in ("wall_ms", "event_ms", "setup_ms", "iters", "relres", "converged")},
-        "mixed": {k: mp[k] for k in ("wall_ms", "event_ms", "setup_ms", "iters", "relres", "converged")},
-        "speedup_vs_plain": round(plain["wall_ms"] / head["wall_ms"], 3),
-        "roofline": {
-            "bound": "hbm", "kernel": "k_smooth<double,3> (finest-level colour pass, FP64 path)",
-            "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "avg_launch_ms": round(avg_ms, 4), "launches": pr["launches"],
-            "algorithmic_bytes_per_launch": pr["bytes"],
-        },
-        "kernels": {"fp64": plain["prof"], args.mp: mp["prof"]},
-    }
+    out = {"metric": METRIC, "unit": "ms/solve", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "data": "synthetic"}
+    if args.workload == "to":
+        plain, learned, keep, extra = run_to(args, prob)
+        value = learned["median_ms"]
+        out.update({
+            "value": round(value, 3), "ms_per_step": round(value, 3), "dtype": f"f64+{args.mp}",
+            "config": {
+                "workload": f"3D {args.n}^3 topology optimization (channel BCs, volume fraction {args.volfrac}, "
+                            f"filter radius 1.5, OC): forward solves of design iterations "
+                            f"{args.warmup}..{args.warmup + args.steps - 1}, rtol {args.rtol} true FP64 residual",
+                "n": args.n, "dofs": int(prob.fixed.size), "rtol": args.rtol, "mp": args.mp,
+                "train_iterations": args.warmup, "epochs_per_iteration": args.epochs,
+            },
+            "plain_fp64": plain, "learned_mp": learned, "keep_mp": keep,
+            "speedup_vs_plain": round(plain["median_ms"] / value, 3),
+            "speedup_keep_vs_plain": round(plain["median_ms"] / keep["median_ms"], 3),
+            "learned": {k: extra[k] for k in ("train_ms_total", "train_ms_per_iter", "inference_ms_median",
+                                              "learned_taken")},
+            "final_objective": extra["final_J"], "final_volume": extra["final_vol"],
+            "roofline": roofline_of(extra["prof"]), "kernels_plain_fp64": extra["prof"],
+        })
+        base_design, base_iters = extra["last_design"], int(np.median(plain["iters"]))
+        log(args, f"[bench] plain {plain['median_ms']:.1f} ms, keep+{args.mp} {keep['median_ms']:.1f} ms, "
+                  f"learned+{args.mp} {learned['median_ms']:.1f} ms (median of {args.steps})")
+    else:
+        res, rho = run_grf(args, prob)
+        p, pp = res["fp64"]
+        m, mp = res[args.mp]
+        value = m["median_ms"]
+        out.update({
+            "value": round(value, 3), "ms_per_step": round(value, 3), "dtype": f"f64+{args.mp}",
+            "config": {"workload": f"3D {args.n}^3 single Stokes/Brinkman solve (channel BCs, GRF density mean "
+                                   f"{args.mean}, seed 12345), rtol {args.rtol} true FP64 residual",
+                       "n": args.n, "dofs": int(prob.fixed.size), "rtol": args.rtol, "mp": args.mp},
+            "plain_fp64": p, "mixed": m, "speedup_vs_plain": round(p["median_ms"] / value, 3),
+            "roofline": roofline_of(pp), "kernels": {"fp64": pp, args.mp: mp},
+        })
+        base_design, base_iters = rho, int(np.median(p["iters"]))
+        log(args, f"[bench] fp64 {p['median_ms']:.1f} ms, {args.mp} {m['median_ms']:.1f} ms")
     if not args.no_cpu_baseline:
         log(args, "[bench] cpu baseline (oracle) ...")
-        out["cpu_baseline"] = cpu_baseline(args, prob, rho, plain["iters"])
+        out["cpu_baseline"] = cpu_baseline(args, prob, base_design, base_iters)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)

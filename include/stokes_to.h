@@ -42,6 +42,7 @@ enum {
 
 enum { ST_FP64 = 0, ST_FP32 = 1, ST_BF16 = 2 };        /* preconditioner precision */
 enum { ST_INIT_ZERO = 0, ST_INIT_KEEP = 1, ST_INIT_HOST = 2, ST_INIT_LEARNED = 3 };
+enum { ST_OPT_OC = 0, ST_OPT_MMA = 1 };                 /* design update of st_to_step */
 
 typedef struct {
     int dim;             /* 2 or 3 */
@@ -57,6 +58,7 @@ typedef struct {
     double omega;        /* smoother damping */
     int closed;          /* add the all-ones pressure gauge on the coarsest level */
     double filter_radius;/* density filter radius in cells (<= 1 disables) */
+    int optimizer;       /* ST_OPT_OC (default) or ST_OPT_MMA, the reference's optimizer */
 } st_params;
 
 typedef struct {
@@ -66,7 +68,8 @@ typedef struct {
     double relres0;      /* true relative residual of the initial guess */
     double ms;           /* wall time of the solve incl. hierarchy setup (hipEvents) */
     double setup_ms;     /* hierarchy setup part (coarsening + coarsest inverse) */
-    double init_ms;      /* initial-guess part (learned predictor inference) */
+    double init_ms;      /* initial-guess part (learned predictor inference and its residual check) */
+    int learned_used;    /* ST_INIT_LEARNED: 1 if the network's candidate had the smaller true residual */
 } st_stats;
 
 /* defaults: the configs' material model (alpha_max 1e4, q 0.1, power 1, scale 1/n^2),
@@ -103,7 +106,11 @@ int st_set_rhs(st_ctx *c, const double *body, const double *dirichlet);
    CX/GMRES.cpp:20-110): FGMRES right-preconditioned by one V-cycle, stopped on the
    true FP64 residual.  init: ST_INIT_ZERO (Dirichlet values, zero elsewhere),
    ST_INIT_KEEP (the device solution of the previous solve), ST_INIT_HOST (x0),
-   ST_INIT_LEARNED (the learned predictor).  x_out (nullable) receives the solution. */
+   ST_INIT_LEARNED (the learned predictor: with a previous solution, that solution
+   corrected by the predicted change of the coarse state between its design and this
+   one; without one, the Dirichlet values plus the predicted coarse state; taken only
+   if its true FP64 residual is below that of the candidate without the network, see
+   st_stats.learned_used).  x_out (nullable) receives the solution. */
 int st_solve(st_ctx *c, int init, const double *x0, double *x_out, st_stats *stats);
 int st_get_solution(st_ctx *c, double *x);
 /* replaces StokeFlowMapping3DFixed::applyMapping through LinearMapping
@@ -126,12 +133,26 @@ int st_filter(st_ctx *c, const double *in, double *out, int transpose);
    volume measured on the filtered density */
 int st_oc_update(st_ctx *c, const double *rho, const double *dc, double volfrac, double move, double eta,
                  double rho_min, double *rho_new, double *change);
+/* replaces MMASolver::Update with one constraint (EXT/mma/MMASolver.cpp:74-89, the
+   update of SX/optimization/OptimizerMma.h:138-170): one method-of-moving-asymptotes
+   step of x (host, ncell values, in place) from the objective gradient dfdx, the
+   constraint value g (feasible if g <= 0) and its gradient dgdx, inside the box
+   [xmin, xmax]; the dual is solved by the same interior-point Newton iteration.
+   The context keeps the asymptotes and the two previous iterates across calls, like
+   the reference's MMASolver object; st_mma_reset starts over. */
+int st_mma_update(st_ctx *c, double *x, const double *dfdx, double g, const double *dgdx, const double *xmin,
+                  const double *xmax, int *newton_steps);
+int st_mma_reset(st_ctx *c);
 /* one full design iteration on the device, the body of Optimize_MMA's loop
    (FT/FluidTopoOpt3D.cpp:4-17): filter, design->alpha, forward solve, objective,
-   adjoint solve, sensitivity, filter transpose, OC update of the device design.
+   adjoint solve, sensitivity, filter transpose, and the design update of the device
+   design: OC, or with params.optimizer = ST_OPT_MMA the reference's MMA step with the
+   reference's volume constraint 1e4 (mean(rho~) - volfrac) <= 0 and box
+   [max(rho_min, rho - move), min(1, rho + move)] (eta unused).
    init: ST_INIT_ZERO (both solves from zero, as the reference's GMRES), ST_INIT_KEEP
-   (previous forward / adjoint solutions), ST_INIT_LEARNED (previous solution plus the
-   learned coarse-grid correction; needs a trained predictor, 3D only) */
+   (previous forward / adjoint solutions), ST_INIT_LEARNED (forward solve from the
+   learned initial guess of st_solve, adjoint from the previous adjoint; needs a
+   trained predictor, 3D only) */
 int st_to_step(st_ctx *c, int init, double volfrac, double move, double eta, double rho_min, double *J,
                double *vol, double *change, st_stats *fwd, st_stats *adj);
 

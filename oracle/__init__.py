@@ -72,6 +72,12 @@ def lib():
         L.orc_conv3d.argtypes = [I, I, I, I, D, D, D, D, I]
         L.orc_conv3d_bwd_data.argtypes = [I, I, I, I, D, D, D]
         L.orc_conv3d_bwd_weight.argtypes = [I, I, I, I, D, D, D, D]
+        L.orc_mma_create.restype = ctypes.c_void_p
+        L.orc_mma_create.argtypes = [ctypes.c_long]
+        L.orc_mma_free.argtypes = [ctypes.c_void_p]
+        L.orc_mma_update.restype = I
+        L.orc_mma_update.argtypes = [ctypes.c_void_p, D, D, ctypes.c_double, D, D, D]
+        L.orc_mma_state.argtypes = [ctypes.c_void_p, D, D, D]
         L.orc_fgmres.restype = I
         L.orc_fgmres.argtypes = [ctypes.c_void_p, D, D, ctypes.c_double, I, I, D, D]
     return _lib
@@ -329,9 +335,11 @@ class MG:
         return x, it, rr.value, hist[: it + 1]
 
 
-def to_step(P, rho, radius=1.5, volfrac=0.3, move=0.2, eta=0.5, rho_min=1e-3, rtol=1e-10, x_prev=None):
-    """One density-filtered OC design iteration, composed from the oracle pieces in the
+def to_step(P, rho, radius=1.5, volfrac=0.3, move=0.2, eta=0.5, rho_min=1e-3, rtol=1e-10, x_prev=None, mma=None):
+    """One density-filtered design iteration, composed from the oracle pieces in the
     order of FT/FluidTopoOpt3D.cpp:4-17 (objective -> gradient via adjoint -> update).
+    The update is OC, or with `mma` (an oracle MMA object) the MMA step with the
+    reference's volume constraint 1e4 (mean(H rho) - volfrac) <= 0 and move box.
     Returns (rho_new, J, change, x, lam)."""
     dim, shape = P.dim, P.shape
     n, s = sizes(dim, *shape)
@@ -351,5 +359,86 @@ def to_step(P, rho, radius=1.5, volfrac=0.3, move=0.2, eta=0.5, rho_min=1e-3, rt
     lam, _, _, _ = mg.fgmres(gx, rtol=rtol)
     dj = sensitivity(dim, shape, rhof, am, P.fixed[nc:], x[nc:], lam[nc:])
     dc = filt(dim, shape, radius, dj, transpose=True) if radius > 1.0 else dj
-    rn, ch = oc_update(dim, shape, radius, rho, dc, volfrac, move, eta, rho_min)
-    return rn, J, ch, x, lam
+    if mma is None:
+        rn, ch = oc_update(dim, shape, radius, rho, dc, volfrac, move, eta, rho_min)
+        return rn, J, ch, x, lam
+    nc_ = rho.size
+    dv = np.full(nc_, 1.0 / nc_)
+    if radius > 1.0:
+        dv = filt(dim, shape, radius, dv, transpose=True)
+    coef = 1e4
+    lo, hi = mma_bounds(rho, move, rho_min, 1.0)
+    rn = _f64(rho).copy()
+    mma.update(rn, dc, coef * (rho @ dv - volfrac), coef * dv, lo, hi)
+    return rn, J, np.abs(rn - rho).max(), x, lam
+
+
+# ------------------------------------------------------------------ MMA
+
+class MMA:
+    """The oracle's MMA (one constraint); `update` changes x in place like the
+    reference's MMASolver::Update (EXT/mma/MMASolver.cpp:74-89)."""
+
+    def __init__(self, n):
+        self.n = n
+        self.h = lib().orc_mma_create(n)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_mma_free(self.h)
+            self.h = None
+
+    def update(self, x, df, g, dg, xmin, xmax):
+        assert x.dtype == np.float64 and x.flags.c_contiguous and x.size == self.n
+        return lib().orc_mma_update(self.h, _d(x), _d(_f64(df)), float(g), _d(_f64(dg)), _d(_f64(xmin)),
+                                    _d(_f64(xmax)))
+
+    def state(self):
+        lm = np.zeros(2)
+        lo = np.zeros(self.n)
+        up = np.zeros(self.n)
+        lib().orc_mma_state(self.h, _d(lm), _d(lo), _d(up))
+        return lm, lo, up
+
+
+_REF_MMA = os.path.join(_HERE, "_ref", "libmma_ref.so")
+
+
+def mma_ref_available():
+    return os.path.exists(_REF_MMA)
+
+
+class MMARef:
+    """The reference's own MMASolver (compiled from its source by `make ref`)."""
+    _l = None
+
+    def __init__(self, n, m=1):
+        if MMARef._l is None:
+            L = ctypes.CDLL(_REF_MMA)
+            L.mmaref_create.restype = ctypes.c_void_p
+            L.mmaref_create.argtypes = [I, I]
+            L.mmaref_free.argtypes = [ctypes.c_void_p]
+            L.mmaref_update.argtypes = [ctypes.c_void_p, D, D, D, D, D, D]
+            MMARef._l = L
+        self.n, self.m = n, m
+        self.h = MMARef._l.mmaref_create(n, m)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            MMARef._l.mmaref_free(self.h)
+            self.h = None
+
+    def update(self, x, df, g, dg, xmin, xmax):
+        gg = np.atleast_1d(_f64(g))
+        MMARef._l.mmaref_update(self.h, _d(x), _d(_f64(df)), _d(gg), _d(_f64(dg)), _d(_f64(xmin)), _d(_f64(xmax)))
+
+
+def mma_bounds(x, move, lb, ub):
+    """Outer move limits of OptimizerMma::Optimize_MMA (SX/optimization/OptimizerMma.h:157-164)."""
+    return np.maximum(lb, x - move), np.minimum(ub, x + move)
+
+
+def volume_constraint(x, frac, coef=1e4):
+    """g = coef (mean x - frac), dg/dx = coef / n (FT/FluidTopoOpt3D.cpp:19-39)."""
+    n = x.size
+    return coef * (x.sum() / n - frac), np.full(n, coef / n)

@@ -1026,3 +1026,181 @@ void orc_conv3d_bwd_weight(int R, int ntap, int Cin, int Cout, const double *in,
         }
     }
 }
+
+/* ------------------------------------------------------------------ MMA (m = 1)
+ * Svanberg's method of moving asymptotes for min f(x) s.t. g(x) <= 0 (one
+ * constraint), x in [xmin, xmax], with the subproblem solved through its dual by
+ * a primal-dual interior-point Newton iteration on (lambda, mu) -- the variant of
+ * EXT/mma/MMASolver.cpp:74-469 (GenSub :360-443, SolveDIP :92-145, XYZofLAMBDA
+ * :332-358, DualGrad :322-330, DualHess :224-320, DualLineSearch :196-222,
+ * DualResidual :165-194) with its constants (a = 0, c = 1000, d = 0, raa0 = 1e-5,
+ * move 0.5, albefa 0.1, asymptotes 0.5 / 0.7 / 1.2, epsimin = 1e-9 sqrt(n + 1)).
+ * Sums run serially in index order, like the reference's, so the two agree to
+ * the last bit where the operations match. */
+struct orc_mma {
+    long n;
+    int iter;
+    double *low, *upp, *xo1, *xo2, *alf, *bet, *p0, *q0, *p1, *q1;
+    double lam, mu, y, z, b;
+};
+
+orc_mma *orc_mma_create(long n)
+{
+    orc_mma *s = (orc_mma *)calloc(1, sizeof(orc_mma));
+    s->n = n;
+    double **arr[10] = {&s->low, &s->upp, &s->xo1, &s->xo2, &s->alf, &s->bet, &s->p0, &s->q0, &s->p1, &s->q1};
+    for (int a = 0; a < 10; a++) *arr[a] = (double *)calloc((size_t)n, sizeof(double));
+    return s;
+}
+
+void orc_mma_free(orc_mma *s)
+{
+    if (!s) return;
+    double *arr[10] = {s->low, s->upp, s->xo1, s->xo2, s->alf, s->bet, s->p0, s->q0, s->p1, s->q1};
+    for (int a = 0; a < 10; a++) free(arr[a]);
+    free(s);
+}
+
+static double mma_max(double a, double b) { return a > b ? a : b; }
+static double mma_min(double a, double b) { return a < b ? a : b; }
+
+/* the subproblem's separable coefficients for one constraint */
+static void mma_subproblem(orc_mma *s, const double *x, const double *df, double g, const double *dg,
+                           const double *xmin, const double *xmax)
+{
+    const double eps = 1.0e-5, raa0 = 1.0e-5, albefa = 0.1, movel = 0.5;
+    s->iter++;
+    for (long i = 0; i < s->n; i++) {
+        const double w = xmax[i] - xmin[i];
+        if (s->iter < 3) {
+            s->low[i] = x[i] - 0.5 * w;
+            s->upp[i] = x[i] + 0.5 * w;
+        } else {
+            const double osc = (x[i] - s->xo1[i]) * (s->xo1[i] - s->xo2[i]);
+            const double gam = osc < 0.0 ? 0.7 : (osc > 0.0 ? 1.2 : 1.0);
+            double L = x[i] - gam * (s->xo1[i] - s->low[i]);
+            double U = x[i] + gam * (s->upp[i] - s->xo1[i]);
+            const double wm = mma_max(eps, w);
+            L = mma_min(mma_max(L, x[i] - 100.0 * wm), x[i] - 1.0e-5 * wm);
+            U = mma_min(mma_max(U, x[i] + 1.0e-5 * wm), x[i] + 100.0 * wm);
+            const double lo = xmin[i] - 1.0e-6, hi = xmax[i] + 1.0e-6;
+            if (x[i] < lo) {
+                L = x[i] - (hi - x[i]) / 0.9;
+                U = x[i] + (hi - x[i]) / 0.9;
+            }
+            if (x[i] > hi) {
+                L = x[i] - (x[i] - lo) / 0.9;
+                U = x[i] + (x[i] - lo) / 0.9;
+            }
+            s->low[i] = L;
+            s->upp[i] = U;
+        }
+        const double L = s->low[i], U = s->upp[i];
+        s->alf[i] = mma_min(mma_max(mma_max(xmin[i], L + albefa * (x[i] - L)), x[i] - movel * w), xmax[i]);
+        s->bet[i] = mma_max(mma_min(mma_min(xmax[i], U - albefa * (U - x[i])), x[i] + movel * w), xmin[i]);
+        const double winv = 1.0 / mma_max(eps, w);
+        const double du = pow(U - x[i], 2.0), dl = pow(x[i] - L, 2.0);
+        double pq = 0.001 * fabs(df[i]) + raa0 * winv;
+        s->p0[i] = du * (mma_max(0.0, df[i]) + pq);
+        s->q0[i] = dl * (mma_max(0.0, -df[i]) + pq);
+        pq = 0.001 * fabs(dg[i]) + raa0 * winv;
+        s->p1[i] = du * (mma_max(0.0, dg[i]) + pq);
+        s->q1[i] = dl * (mma_max(0.0, -dg[i]) + pq);
+    }
+    double b = -g;
+    for (long i = 0; i < s->n; i++) b += s->p1[i] / (s->upp[i] - x[i]) + s->q1[i] / (x[i] - s->low[i]);
+    s->b = b;
+}
+
+/* primal point of the dual at the current lambda (clamped to [alpha, beta]) */
+static void mma_primal(orc_mma *s, double *x)
+{
+    const double cc = 1000.0;
+    if (s->lam < 0.0) s->lam = 0.0;
+    s->y = mma_max(0.0, s->lam - cc);
+    s->z = mma_max(0.0, 10.0 * (s->lam * 0.0 - 1.0));
+    for (long i = 0; i < s->n; i++) {
+        const double P = s->p0[i] + s->p1[i] * s->lam, Q = s->q0[i] + s->q1[i] * s->lam;
+        double v = (sqrt(P) * s->low[i] + sqrt(Q) * s->upp[i]) / (sqrt(P) + sqrt(Q));
+        if (v < s->alf[i]) v = s->alf[i];
+        if (v > s->bet[i]) v = s->bet[i];
+        x[i] = v;
+    }
+}
+
+/* dual gradient -b - a z - y (+ extra) + sum(p1/(U-x) + q1/(x-L)); the KKT residual
+ * adds mu before the sum, in the reference's order */
+static double mma_dual_grad(const orc_mma *s, const double *x, double extra)
+{
+    double gsum = -s->b - 0.0 * s->z - s->y + extra;
+    for (long i = 0; i < s->n; i++) gsum += s->p1[i] / (s->upp[i] - x[i]) + s->q1[i] / (x[i] - s->low[i]);
+    return gsum;
+}
+
+static double mma_dual_hess(orc_mma *s, const double *x)
+{
+    const double cc = 1000.0;
+    double h = 0.0;
+    for (long i = 0; i < s->n; i++) {
+        const double P = s->p0[i] + s->p1[i] * s->lam, Q = s->q0[i] + s->q1[i] * s->lam;
+        const double du = s->upp[i] - x[i], dl = x[i] - s->low[i];
+        const double pq = s->p1[i] / pow(du, 2.0) - s->q1[i] / pow(dl, 2.0);
+        double d2 = -1.0 / (2.0 * P / pow(du, 3.0) + 2.0 * Q / pow(dl, 3.0));
+        const double xp = (sqrt(P) * s->low[i] + sqrt(Q) * s->upp[i]) / (sqrt(P) + sqrt(Q));
+        if (xp < s->alf[i] || xp > s->bet[i]) d2 = 0.0;
+        h += pq * d2 * pq;
+    }
+    if (s->lam < 0.0) s->lam = 0.0;
+    if (s->lam > cc) h += -1.0;
+    h += -s->mu / s->lam;
+    double corr = 1e-4 * h;
+    if (-corr < 1.0e-7) corr = -1.0e-7;
+    return h + corr;
+}
+
+/* one MMA update of x (in place); returns the number of Newton steps */
+int orc_mma_update(orc_mma *s, double *x, const double *df, double g, const double *dg, const double *xmin,
+                   const double *xmax)
+{
+    mma_subproblem(s, x, df, g, dg, xmin, xmax);
+    memcpy(s->xo2, s->xo1, sizeof(double) * (size_t)s->n);
+    memcpy(s->xo1, x, sizeof(double) * (size_t)s->n);
+    const double tol = sqrt((double)s->n + 1.0) * 1e-9;
+    s->lam = 1000.0 / 2.0;
+    s->mu = 1.0;
+    double epsi = 1.0, err = 1.0;
+    int steps = 0;
+    while (epsi > tol) {
+        int loop = 0;
+        while (err > 0.9 * epsi && loop < 100) {
+            loop++;
+            steps++;
+            mma_primal(s, x);
+            const double gr = -mma_dual_grad(s, x, 0.0) - epsi / s->lam;
+            const double h = mma_dual_hess(s, x);
+            const double s0 = gr / h;
+            const double s1 = -s->mu + epsi / s->lam - s0 * s->mu / s->lam;
+            double th = 1.005;
+            if (th < -1.01 * s0 / s->lam) th = -1.01 * s0 / s->lam;
+            if (th < -1.01 * s1 / s->mu) th = -1.01 * s1 / s->mu;
+            th = 1.0 / th;
+            s->lam += th * s0;
+            s->mu += th * s1;
+            mma_primal(s, x);
+            const double r0 = mma_dual_grad(s, x, s->mu), r1 = s->mu * s->lam - epsi;
+            err = fabs(r0) > fabs(r1) ? fabs(r0) : fabs(r1);
+        }
+        epsi *= 0.1;
+    }
+    return steps;
+}
+
+void orc_mma_state(const orc_mma *s, double *lam_mu, double *low, double *upp)
+{
+    if (lam_mu) {
+        lam_mu[0] = s->lam;
+        lam_mu[1] = s->mu;
+    }
+    if (low) memcpy(low, s->low, sizeof(double) * (size_t)s->n);
+    if (upp) memcpy(upp, s->upp, sizeof(double) * (size_t)s->n);
+}

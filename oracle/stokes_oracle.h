@@ -92,6 +92,14 @@ void orc_conv3d(int R, int ntap, int Cin, int Cout, const double *in, const doub
 void orc_conv3d_bwd_data(int R, int ntap, int Cin, int Cout, const double *gout, const double *W, double *gin);
 void orc_conv3d_bwd_weight(int R, int ntap, int Cin, int Cout, const double *in, const double *gout, double *gW,
                            double *gb);
+/* MMA with one constraint (EXT/mma/MMASolver.cpp restated for m = 1): state holds
+ * the asymptotes and the two previous iterates */
+typedef struct orc_mma orc_mma;
+orc_mma *orc_mma_create(long n);
+void orc_mma_free(orc_mma *s);
+int orc_mma_update(orc_mma *s, double *x, const double *df, double g, const double *dg, const double *xmin,
+                   const double *xmax);
+void orc_mma_state(const orc_mma *s, double *lam_mu, double *low, double *upp);
 /* FGMRES(m), right-preconditioned by one V-cycle; returns iterations, relres out */
 int orc_fgmres(orc_mg *mg, const double *b, double *x, double rtol, int restart, int maxit,
                double *relres, double *history);

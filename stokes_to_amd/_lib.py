@@ -14,6 +14,7 @@ DEFAULT_LIB = os.path.join(HERE, "libstokes_hip.so")
 ST_OK, ST_EINVAL, ST_EHIP, ST_ENOCONV, ST_ESINGULAR, ST_ESTATE, ST_ENAN = 0, -1, -2, -3, -4, -5, -6
 ST_FP64, ST_FP32, ST_BF16 = 0, 1, 2
 ST_INIT_ZERO, ST_INIT_KEEP, ST_INIT_HOST, ST_INIT_LEARNED = 0, 1, 2, 3
+ST_OPT_OC, ST_OPT_MMA = 0, 1
 
 ERRNAMES = {ST_EINVAL: "EINVAL", ST_EHIP: "EHIP", ST_ENOCONV: "ENOCONV", ST_ESINGULAR: "ESINGULAR",
             ST_ESTATE: "ESTATE", ST_ENAN: "ENAN"}
@@ -27,6 +28,7 @@ class StParams(ctypes.Structure):
         ("precision", ctypes.c_int), ("rtol", ctypes.c_double), ("max_iter", ctypes.c_int),
         ("restart", ctypes.c_int), ("nu_pre", ctypes.c_int), ("nu_post", ctypes.c_int),
         ("omega", ctypes.c_double), ("closed", ctypes.c_int), ("filter_radius", ctypes.c_double),
+        ("optimizer", ctypes.c_int),
     ]
 
 
@@ -34,7 +36,7 @@ class StStats(ctypes.Structure):
     _fields_ = [
         ("iters", ctypes.c_int), ("converged", ctypes.c_int), ("relres", ctypes.c_double),
         ("relres0", ctypes.c_double), ("ms", ctypes.c_double), ("setup_ms", ctypes.c_double),
-        ("init_ms", ctypes.c_double),
+        ("init_ms", ctypes.c_double), ("learned_used", ctypes.c_int),
     ]
 
     def as_dict(self):
@@ -72,6 +74,8 @@ SIGNATURES = {
     "st_sensitivity": (I, [C, D, D, D, D]),
     "st_filter": (I, [C, D, D, I]),
     "st_oc_update": (I, [C, D, D, F, F, F, F, D, D]),
+    "st_mma_update": (I, [C, D, D, F, D, D, D, ctypes.POINTER(ctypes.c_int)]),
+    "st_mma_reset": (I, [C]),
     "st_to_step": (I, [C, I, F, F, F, F, D, D, D, ctypes.POINTER(StStats), ctypes.POINTER(StStats)]),
     "st_record_sample": (I, [C]),
     "st_num_samples": (I, [C]),

@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libstokes_hip.so")
-SOURCES = ["st_kernels.hip", "st_march.hip", "st_solver.hip", "st_net.hip"]
+SOURCES = ["st_kernels.hip", "st_march.hip", "st_solver.hip", "st_net.hip", "st_mma.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("ST_OFFLOAD_ARCH", "gfx950")
 

@@ -12,6 +12,7 @@
 //    and orthogonalises on the host, CX/GMRES.cpp:46-95).
 #include "../../include/stokes_to.h"
 #include "st_kernels.h"
+#include "st_mma.h"
 #include "st_net.h"
 
 #include <cmath>
@@ -54,6 +55,9 @@ struct st_ctx {
     int npart = 0;
     // design
     double *rho = nullptr, *rhof = nullptr, *dj = nullptr, *dc = nullptr, *rn = nullptr, *fW = nullptr, *dv = nullptr;
+    double *tmpc = nullptr;  // cell scratch of the OC update
+    MmaDev *mma = nullptr;   // MMA state (created on first use)
+    double *mlo = nullptr, *mhi = nullptr, *mdg = nullptr;
     // staging
     double *abi_a = nullptr, *abi_b = nullptr;
     uint8_t *abi_u8 = nullptr;
@@ -66,8 +70,9 @@ struct st_ctx {
     NetImpl *net = nullptr;     // learned predictor (created on first use)
     int lc = -1;                // network level (n/4)
     const double *rho_used = nullptr;  // density the current alpha was computed from
-    double *xdc = nullptr, *xcp = nullptr, *xcn = nullptr;  // level-lc Dirichlet state, previous / new prediction
-    bool have_pred = false, have_lam = false;
+    double *xdc = nullptr, *xcp = nullptr, *xcn = nullptr;  // level-lc Dirichlet state, two predictions
+    double *rhop = nullptr;     // design of the solution held in X (for the learned correction)
+    bool have_rhop = false, have_lam = false;
     // kernel timing: event pairs around finest-level launches
     bool prof = false;
     std::vector<hipEvent_t> evpool[ST_PROF_NKIND];
@@ -369,6 +374,7 @@ static int fgmres(st_ctx *c, const double *b, double *x, st_stats *st)
     }
     if (st) {
         st->iters = it;
+        st->learned_used = 0;
         st->relres = rr;
         st->relres0 = rr0;
         st->converged = rr <= rtol;
@@ -418,6 +424,7 @@ void st_default_params(st_params *p, int dim, int n)
     p->omega = 1.0;
     p->closed = 0;
     p->filter_radius = 1.5;
+    p->optimizer = ST_OPT_OC;
 }
 
 int st_create(st_ctx **out, const st_params *p)
@@ -485,7 +492,7 @@ int st_create(st_ctx **out, const st_params *p)
     if (rc == ST_OK) rc = dalloc(c, &c->part, c->npart);
     if (rc == ST_OK) rc = dalloc(c, &c->red, 64);
     if (rc == ST_OK) rc = dalloc(c, &c->h_dev, 64);
-    for (double **pp : {&c->rho, &c->rhof, &c->dj, &c->dc, &c->rn, &c->fW, &c->dv})
+    for (double **pp : {&c->rho, &c->rhof, &c->dj, &c->dc, &c->rn, &c->fW, &c->dv, &c->tmpc})
         if (rc == ST_OK) rc = dalloc(c, pp, c->ncell);
     if (rc == ST_OK) rc = dalloc(c, &c->abi_a, c->ndof);
     if (rc == ST_OK) rc = dalloc(c, &c->abi_b, c->ndof);
@@ -527,6 +534,7 @@ void st_destroy(st_ctx *c)
     if (!c) return;
     if (c->s) hipStreamSynchronize(c->s);
     if (c->net) net_destroy(c->net);
+    if (c->mma) mma_destroy(c->mma);
     for (void *q : c->allocs) hipFree(q);
     if (c->e0) hipEventDestroy(c->e0);
     if (c->e1) hipEventDestroy(c->e1);
@@ -713,6 +721,7 @@ static int net_ready(st_ctx *c)
     CKR(dalloc(c, &c->xdc, nv));
     CKR(dalloc(c, &c->xcp, nv));
     CKR(dalloc(c, &c->xcn, nv));
+    CKR(dalloc(c, &c->rhop, c->ncell));
     c->net = net_create(gc.nx, 32, 12345u, c->s);
     if (!c->net) return fail(c, ST_EHIP, "cannot allocate the predictor");
     return ST_OK;
@@ -734,20 +743,49 @@ static void prolong_state_add(st_ctx *c, const double *xc, double *x)
     launch_prolong<double>(c->lv[0], c->lv[1], c->fl[0], c->h64.t[1], x, c->s);
 }
 
-static int predict(st_ctx *c, double *xc)
+static int predict(st_ctx *c, const double *rho_dev, double *xc)
 {
-    if (!c->rho_used) return fail(c, ST_ESTATE, "the predictor needs a design (st_set_design / st_to_step)");
+    if (!rho_dev) return fail(c, ST_ESTATE, "the predictor needs a design (st_set_design / st_to_step)");
     restrict_state(c, c->XD, c->xdc);
-    if (net_predict(c->net, c->lv[c->lc], c->n, c->n / c->lv[c->lc].nx, c->rho_used, c->fl[c->lc], c->xdc, xc) != 0)
+    if (net_predict(c->net, c->lv[c->lc], c->n, c->n / c->lv[c->lc].nx, rho_dev, c->fl[c->lc], c->xdc, xc) != 0)
         return fail(c, ST_EINVAL, "predictor/level mismatch");
     return ST_OK;
 }
 
-// Learned initial guess in X, returns its device time in ms.  After a learned solve
-// of a previous design: X = X_prev + P P (net(rho) - net(rho_prev)), the previous
-// solution corrected by the predicted change of the coarse state; after another
-// solve: X = X_prev; with no solution yet: X = x_D + P P net(rho).
-static int learned_guess(st_ctx *c, float *ms)
+// true FP64 residual norm ||B - A x|| (scratch: the first Krylov vector)
+static int resid_norm(st_ctx *c, const double *x, double *nrm)
+{
+    const Lvl &g = c->lv[0];
+    launch_residual<double>(g, c->fl[0], c->h64.al[0], c->B, x, c->V, c->part, c->s);
+    launch_sum((int)residual_blocks(g), 1, c->part, c->red, c->s);
+    double r2 = 0.0;
+    CK(hipMemcpyAsync(&r2, c->red, sizeof(double), hipMemcpyDeviceToHost, c->s));
+    CK(hipStreamSynchronize(c->s));
+    *nrm = std::sqrt(r2);
+    return ST_OK;
+}
+
+// after a forward solve: remember the design its solution belongs to
+static void remember_design(st_ctx *c)
+{
+    if (c->net && c->rhop && c->rho_used) {
+        launch_axpby<double, double>(c->ncell, 1.0, c->rho_used, 0.0, c->rhop, c->s);
+        c->have_rhop = true;
+    } else {
+        c->have_rhop = false;
+    }
+}
+
+// Learned initial guess in X; returns its device time in ms and whether the
+// network's candidate was taken.  Candidates (both network evaluations use the
+// current weights):
+//   with the solution of a previous design in X:
+//     X + P P (net(rho) - net(rho_prev))  against  X (the previous solution);
+//   without one:
+//     x_D + P P net(rho)                   against  x_D (the zero initial guess).
+// The candidate with the smaller true FP64 residual ||b - A x|| is kept, so the
+// plain method remains the judge of every prediction.
+static int learned_guess(st_ctx *c, float *ms, int *used)
 {
     CKR(net_ready(c));
     if (net_num_samples(c->net) < 1) return fail(c, ST_ESTATE, "the predictor is not trained");
@@ -758,17 +796,25 @@ static int learned_guess(st_ctx *c, float *ms)
     CK(hipEventCreate(&a));
     CK(hipEventCreate(&b));
     CK(hipEventRecord(a, c->s));
-    CKR(predict(c, c->xcn));
-    if (c->have_x && c->have_pred) {
-        launch_axpby<double, double>(nvc, -1.0, c->xcp, 1.0, c->xcn, c->s);  // xcn = new - prev
+    const bool corr = c->have_x && c->have_rhop;
+    if (!c->have_x) launch_axpby<double, double>(c->nvec, 1.0, c->XD, 0.0, c->X, c->s);
+    int take = 0;
+    if (c->have_x && !c->have_rhop) {
+        take = 0;  // no design for the held solution: plain warm start
+    } else {
+        CKR(predict(c, c->rho_used, c->xcn));
+        if (corr) {
+            CKR(predict(c, c->rhop, c->xcp));
+            launch_axpby<double, double>(nvc, -1.0, c->xcp, 1.0, c->xcn, c->s);  // xcn = net(rho) - net(rho_prev)
+        }
+        double r_base = 0.0, r_cand = 0.0;
+        CKR(resid_norm(c, c->X, &r_base));
+        launch_axpby<double, double>(c->nvec, 1.0, c->X, 0.0, c->Wv, c->s);  // keep the base
         prolong_state_add(c, c->xcn, c->X);
-        launch_axpby<double, double>(nvc, 1.0, c->xcp, 1.0, c->xcn, c->s);   // restore new
-    } else if (!c->have_x) {
-        launch_axpby<double, double>(c->nvec, 1.0, c->XD, 0.0, c->X, c->s);
-        prolong_state_add(c, c->xcn, c->X);
+        CKR(resid_norm(c, c->X, &r_cand));
+        take = (std::isfinite(r_cand) && r_cand < r_base) ? 1 : 0;
+        if (!take) launch_axpby<double, double>(c->nvec, 1.0, c->Wv, 0.0, c->X, c->s);
     }
-    launch_axpby<double, double>(nvc, 1.0, c->xcn, 0.0, c->xcp, c->s);
-    c->have_pred = true;
     CK(hipEventRecord(b, c->s));
     CK(hipEventSynchronize(b));
     float ims = 0.f;
@@ -776,6 +822,7 @@ static int learned_guess(st_ctx *c, float *ms)
     hipEventDestroy(a);
     hipEventDestroy(b);
     if (ms) *ms = ims;
+    if (used) *used = take;
     return ST_OK;
 }
 
@@ -792,13 +839,18 @@ int st_solve(st_ctx *c, int init, const double *x0, double *x_out, st_stats *sta
         if (!c->have_x) init = ST_INIT_ZERO;
     } else if (init == ST_INIT_LEARNED) {
         float ims = 0.f;
-        CKR(learned_guess(c, &ims));
+        int used = 0;
+        CKR(learned_guess(c, &ims, &used));
         int rc = solve_core(c, c->B, c->X, c->XD, ST_INIT_KEEP, stats);
         if (stats) {
             stats->init_ms = ims;
             stats->ms += ims;
+            stats->learned_used = used;
         }
-        if (rc == ST_OK || rc == ST_ENOCONV) c->have_x = true;
+        if (rc == ST_OK || rc == ST_ENOCONV) {
+            c->have_x = true;
+            remember_design(c);
+        }
         if (x_out && (rc == ST_OK || rc == ST_ENOCONV)) {
             launch_unpack<double>(c->lv[0], c->X, c->abi_b, c->s);
             int r2 = stage_out(c, x_out, c->ndof);
@@ -808,9 +860,11 @@ int st_solve(st_ctx *c, int init, const double *x0, double *x_out, st_stats *sta
     } else if (init != ST_INIT_ZERO) {
         return ST_EINVAL;
     }
-    c->have_pred = false;  // the stored prediction no longer matches the solution
     int rc = solve_core(c, c->B, c->X, c->XD, init == ST_INIT_HOST ? ST_INIT_KEEP : init, stats);
-    if (rc == ST_OK || rc == ST_ENOCONV) c->have_x = true;
+    if (rc == ST_OK || rc == ST_ENOCONV) {
+        c->have_x = true;
+        remember_design(c);
+    }
     if (x_out && (rc == ST_OK || rc == ST_ENOCONV)) {
         launch_unpack<double>(c->lv[0], c->X, c->abi_b, c->s);
         int r2 = stage_out(c, x_out, c->ndof);
@@ -918,8 +972,8 @@ static int oc_dev(st_ctx *c, const double *rho, const double *dc, double volfrac
                   double rho_min, double *rn, double *change, double *vol_out)
 {
     const long n = c->ncell;
-    launch_fill<double>(n, c->rhof, 1.0 / (double)n, c->s);
-    filter_dev(c, c->rhof, c->dv, 1);  // dV/drho = H^T(1/(n W))
+    launch_fill<double>(n, c->tmpc, 1.0 / (double)n, c->s);
+    filter_dev(c, c->tmpc, c->dv, 1);  // dV/drho = H^T(1/(n W))
     const int nb = (int)oc_blocks(n);
     double l1 = 0.0, l2 = 1e9, vol = 0.0;
     int it = 0;
@@ -956,6 +1010,84 @@ int st_oc_update(st_ctx *c, const double *rho, const double *dcp, double volfrac
     return ST_OK;
 }
 
+static int mma_ready(st_ctx *c)
+{
+    if (c->mma) return ST_OK;
+    CKR(dalloc(c, &c->mlo, c->ncell));
+    CKR(dalloc(c, &c->mhi, c->ncell));
+    CKR(dalloc(c, &c->mdg, c->ncell));
+    c->mma = mma_create(c->ncell, c->s);
+    if (!c->mma) return fail(c, ST_EHIP, "cannot allocate the MMA state");
+    return ST_OK;
+}
+
+int st_mma_update(st_ctx *c, double *x, const double *dfdx, double g, const double *dgdx, const double *xmin,
+                  const double *xmax, int *newton_steps)
+{
+    if (!c || !x || !dfdx || !dgdx || !xmin || !xmax) return ST_EINVAL;
+    if (!std::isfinite(g)) return fail(c, ST_ENAN, "non-finite constraint value");
+    CKR(check_finite(c, x, c->ncell, "x"));
+    CKR(check_finite(c, dfdx, c->ncell, "dfdx"));
+    CKR(mma_ready(c));
+    const size_t nb = sizeof(double) * c->ncell;
+    CK(hipMemcpyAsync(c->rn, x, nb, hipMemcpyHostToDevice, c->s));
+    CK(hipMemcpyAsync(c->dc, dfdx, nb, hipMemcpyHostToDevice, c->s));
+    CK(hipMemcpyAsync(c->mdg, dgdx, nb, hipMemcpyHostToDevice, c->s));
+    CK(hipMemcpyAsync(c->mlo, xmin, nb, hipMemcpyHostToDevice, c->s));
+    CK(hipMemcpyAsync(c->mhi, xmax, nb, hipMemcpyHostToDevice, c->s));
+    int r = mma_update(c->mma, c->rn, c->dc, g, c->mdg, c->mlo, c->mhi, newton_steps);
+    if (r == -2) return fail(c, ST_ENAN, "MMA dual iteration produced a non-finite value");
+    if (r != 0) return fail(c, ST_EHIP, "MMA update failed");
+    CK(hipMemcpyAsync(x, c->rn, nb, hipMemcpyDeviceToHost, c->s));
+    CK(hipStreamSynchronize(c->s));
+    return ST_OK;
+}
+
+int st_mma_reset(st_ctx *c)
+{
+    if (!c) return ST_EINVAL;
+    if (c->mma) mma_reset(c->mma);
+    return ST_OK;
+}
+
+// MMA design update of the device design (one volume constraint on the filtered
+// density, scaled like the reference's, FT/FluidTopoOpt3D.cpp:19-39):
+// g = 1e4 (mean(H rho) - volfrac), dg = 1e4 H^T(1/n); box max(rho_min, rho - move) ..
+// min(1, rho + move) (SX/optimization/OptimizerMma.h:157-164)
+static int mma_dev(st_ctx *c, const double *rho, const double *dc, double volfrac, double move, double rho_min,
+                   double *rn, double *change, double *vol_out)
+{
+    const long n = c->ncell;
+    CKR(mma_ready(c));
+    launch_fill<double>(n, c->tmpc, 1.0 / (double)n, c->s);
+    filter_dev(c, c->tmpc, c->dv, 1);  // dV/drho
+    double vol = 0.0;
+    launch_dot<double, double>(n, rho, c->dv, c->part, c->s);
+    launch_sum((int)blas_blocks(n), 1, c->part, c->red, c->s);
+    CK(hipMemcpyAsync(&vol, c->red, sizeof(double), hipMemcpyDeviceToHost, c->s));
+    CK(hipStreamSynchronize(c->s));
+    const double coef = 1e4;
+    launch_axpby<double, double>(n, coef, c->dv, 0.0, c->mdg, c->s);
+    launch_mma_box(n, rho, move, rho_min, 1.0, c->mlo, c->mhi, c->s);
+    launch_axpby<double, double>(n, 1.0, rho, 0.0, rn, c->s);
+    int r = mma_update(c->mma, rn, dc, coef * (vol - volfrac), c->mdg, c->mlo, c->mhi, nullptr);
+    if (r == -2) return fail(c, ST_ENAN, "MMA dual iteration produced a non-finite value");
+    if (r != 0) return fail(c, ST_EHIP, "MMA update failed");
+    launch_absdiff_max(n, rn, rho, c->part, c->s);
+    std::vector<double> pm(absmax_blocks(n));
+    CK(hipMemcpyAsync(pm.data(), c->part, sizeof(double) * pm.size(), hipMemcpyDeviceToHost, c->s));
+    launch_dot<double, double>(n, rn, c->dv, c->part + pm.size(), c->s);
+    launch_sum((int)blas_blocks(n), 1, c->part + pm.size(), c->red, c->s);
+    double vn = 0.0;
+    CK(hipMemcpyAsync(&vn, c->red, sizeof(double), hipMemcpyDeviceToHost, c->s));
+    CK(hipStreamSynchronize(c->s));
+    double ch = 0.0;
+    for (double v : pm) ch = v > ch ? v : ch;
+    if (change) *change = ch;
+    if (vol_out) *vol_out = vn;
+    return ST_OK;
+}
+
 int st_to_step(st_ctx *c, int init, double volfrac, double move, double eta, double rho_min, double *J,
                double *vol, double *change, st_stats *fwd, st_stats *adj)
 {
@@ -967,17 +1099,18 @@ int st_to_step(st_ctx *c, int init, double volfrac, double move, double eta, dou
     design_to_alpha(c, c->rhof);
     int fmode = (init == ST_INIT_ZERO || !c->have_x) ? ST_INIT_ZERO : ST_INIT_KEEP;
     float ims = 0.f;
+    int used = 0;
     if (init == ST_INIT_LEARNED && c->have_x && c->net && net_num_samples(c->net) > 0)
-        CKR(learned_guess(c, &ims));
-    else
-        c->have_pred = false;
+        CKR(learned_guess(c, &ims, &used));
     int rc = solve_core(c, c->B, c->X, c->XD, fmode, fwd);
     if (fwd) {
         fwd->init_ms = ims;
         fwd->ms += ims;
+        fwd->learned_used = used;
     }
     if (rc != ST_OK) return rc;
     c->have_x = true;
+    remember_design(c);
     CKR(objective_dev(c, c->X, c->BA, J));
     // adjoint: A lam = G x (A symmetric), homogeneous Dirichlet data
     int amode = (init == ST_INIT_ZERO || !c->have_lam) ? ST_INIT_ZERO : ST_INIT_KEEP;
@@ -986,7 +1119,10 @@ int st_to_step(st_ctx *c, int init, double volfrac, double move, double eta, dou
     c->have_lam = true;
     launch_sensitivity(c->lv[0], c->rhof, alpha_params(c), c->fl[0], c->X, c->LAM, c->dj, c->s);
     filter_dev(c, c->dj, c->dc, 1);
-    CKR(oc_dev(c, c->rho, c->dc, volfrac, move, eta, rho_min, c->rn, change, vol));
+    if (c->prm.optimizer == ST_OPT_MMA)
+        CKR(mma_dev(c, c->rho, c->dc, volfrac, move, rho_min, c->rn, change, vol));
+    else
+        CKR(oc_dev(c, c->rho, c->dc, volfrac, move, eta, rho_min, c->rn, change, vol));
     launch_axpby<double, double>(c->ncell, 1.0, c->rn, 0.0, c->rho, c->s);
     CK(hipStreamSynchronize(c->s));
     return ST_OK;

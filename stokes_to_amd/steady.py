@@ -16,6 +16,7 @@ from .problems import dof_sizes
 
 _INIT = {"zero": ST_INIT_ZERO, "keep": ST_INIT_KEEP, "host": ST_INIT_HOST, "learned": ST_INIT_LEARNED}
 _PREC = {"fp64": 0, "fp32": 1, "bf16": 2}
+_OPT = {"oc": 0, "mma": 1}
 
 
 def _f64(a):
@@ -32,6 +33,8 @@ class FluidSteadyState:
         for k, v in params.items():
             if k == "precision" and isinstance(v, str):
                 v = _PREC[v]
+            if k == "optimizer" and isinstance(v, str):
+                v = _OPT[v]
             if not hasattr(p, k):
                 raise TypeError(f"unknown parameter {k}")
             setattr(p, k, v)
@@ -148,6 +151,18 @@ class FluidSteadyState:
         self._check(self.lib.st_oc_update(self.h, dptr(_f64(rho)), dptr(_f64(dc)), float(volfrac), float(move),
                                           float(eta), float(rho_min), dptr(out), ctypes.byref(ch)))
         return out, ch.value
+
+    # ---- EXT/mma/MMASolver.cpp:74-89 ----
+    def mma_update(self, x, dfdx, g, dgdx, xmin, xmax):
+        """One MMA step of x (returned, a new array) with one constraint g(x) <= 0."""
+        x = _f64(x).copy()
+        steps = ctypes.c_int(0)
+        self._check(self.lib.st_mma_update(self.h, dptr(x), dptr(_f64(dfdx)), float(g), dptr(_f64(dgdx)),
+                                           dptr(_f64(xmin)), dptr(_f64(xmax)), ctypes.byref(steps)))
+        return x, steps.value
+
+    def mma_reset(self):
+        self._check(self.lib.st_mma_reset(self.h))
 
     def to_step(self, volfrac, move=0.2, eta=0.5, rho_min=1e-3, init="zero"):
         """One design iteration on the device (FT/FluidTopoOpt3D.cpp:4-17); init is

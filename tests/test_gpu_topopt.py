@@ -29,3 +29,48 @@ def test_to_loop_matches_oracle(make_solver, orc, dim, n, iters, volfrac):
     rho_g = s.get_design()
     assert rel(rho_g, rho_o) < 1e-4
     assert abs(out["vol"] - volfrac) < 1e-8
+
+
+@pytest.mark.parametrize("n", [4096, 262144])
+def test_mma_update_matches_oracle(make_solver, orc, n):
+    """st_mma_update (device MMA, one constraint) against the oracle's MMA, which is
+    bit-identical to the reference's MMASolver (test_oracle.py): same iterates to
+    round-off of the reduction order over 15 updates."""
+    N = round(n ** (1 / 3))
+    s = make_solver(3, N)
+    rng = np.random.default_rng(5)
+    af = rng.uniform(0.5, 2.0, n)
+    bf = rng.uniform(-1.0, 1.0, n)
+    grad = lambda x: 2.0 * af * (x - 0.3) + bf * np.cos(3.0 * x)
+    xo = np.full(n, 0.5)
+    xg = xo.copy()
+    M = orc.MMA(n)
+    for it in range(15):
+        g, dg = orc.volume_constraint(xo, 0.4)
+        lo, hi = orc.mma_bounds(xo, 0.2, 0.01, 1.0)
+        steps_o = M.update(xo, grad(xo), g, dg, lo, hi)
+        g2, dg2 = orc.volume_constraint(xg, 0.4)
+        lo2, hi2 = orc.mma_bounds(xg, 0.2, 0.01, 1.0)
+        xg, steps_g = s.mma_update(xg, grad(xg), g2, dg2, lo2, hi2)
+        assert np.abs(xg - xo).max() < 1e-9, (it, np.abs(xg - xo).max())
+        assert steps_g == steps_o
+
+
+@pytest.mark.parametrize("radius", [1.5, 1.0])
+def test_to_loop_mma_matches_oracle(make_solver, orc, radius):
+    """TO loop with the reference's optimizer (params.optimizer = MMA) against the
+    oracle's loop with its MMA; radius 1.0 is the reference's unfiltered setting."""
+    dim, n, iters, volfrac = 3, 16, 6, 0.25
+    P = channel_problem(dim, n)
+    rho0 = np.full(n ** dim, volfrac)
+    s = solver_for(make_solver, P, rho0, rtol=1e-10, optimizer="mma", filter_radius=radius)
+    rho_o = rho0.copy()
+    M = orc.MMA(rho0.size)
+    for it in range(iters):
+        out = s.to_step(volfrac, move=0.2, rho_min=0.01)
+        rho_o, J_o, ch_o, _, _ = orc.to_step(P, rho_o, radius=radius, volfrac=volfrac, move=0.2, rho_min=0.01,
+                                             rtol=1e-10, mma=M)
+        assert out["fwd"]["converged"] and out["adj"]["converged"]
+        assert abs(out["J"] - J_o) <= 1e-6 * abs(J_o), (it, out["J"], J_o)
+        assert abs(out["change"] - ch_o) <= 1e-6 + 1e-4 * ch_o
+    assert rel(s.get_design(), rho_o) < 1e-4

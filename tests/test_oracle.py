@@ -323,3 +323,57 @@ def test_bf16_rounding_known_values():
     assert abs(r[6] / 1e-40 - 1) < 2 ** -2
     # idempotent
     assert np.array_equal(N.bf16_round(r), r)
+
+
+# ---------------------------------------------------------------- MMA
+
+def _mma_problem(n, seed):
+    rng = np.random.default_rng(seed)
+    af = rng.uniform(0.5, 2.0, n)
+    bf = rng.uniform(-1.0, 1.0, n)
+    return lambda x: 2.0 * af * (x - 0.3) + bf * np.cos(3.0 * x)
+
+
+@pytest.fixture(scope="session")
+def mma_ref(orc):
+    """The reference's own MMASolver.cpp, compiled where it lies (oracle/Makefile `ref`)."""
+    import os
+    import subprocess
+    if not orc.mma_ref_available():
+        if not os.path.isdir("/root/reference/simplex/ext/mma"):
+            pytest.skip("reference sources not present (MMA compared against the compiled reference here only)")
+        subprocess.check_call(["make", "-s", "-C", os.path.dirname(orc.__file__), "ref"])
+    return orc.MMARef
+
+
+@pytest.mark.parametrize("n,seed", [(500, 0), (4096, 1)])
+def test_mma_restatement_matches_compiled_reference(orc, mma_ref, n, seed):
+    """The oracle's MMA against the reference's MMASolver (one volume constraint,
+    outer move box of OptimizerMma.h:157-164): identical iterates, bit for bit."""
+    grad = _mma_problem(n, seed)
+    xa = np.full(n, 0.5)
+    xb = xa.copy()
+    A, B = orc.MMA(n), mma_ref(n)
+    for it in range(25):
+        for x, M in ((xa, A), (xb, B)):
+            g, dg = orc.volume_constraint(x, 0.4)
+            lo, hi = orc.mma_bounds(x, 0.2, 0.01, 1.0)
+            M.update(x, grad(x), g, dg, lo, hi)
+        assert np.array_equal(xa, xb), (it, np.abs(xa - xb).max())
+
+
+def test_mma_step_is_feasible_and_inside_the_box(orc):
+    """Properties fixed by the method: every iterate stays inside the move box and
+    the (linear) volume constraint ends up satisfied once it is active."""
+    n = 1000
+    grad = lambda x: -np.ones(n) - 0.1 * x      # objective prefers large x -> constraint active
+    x = np.full(n, 0.2)
+    M = orc.MMA(n)
+    for it in range(30):
+        g, dg = orc.volume_constraint(x, 0.4)
+        lo, hi = orc.mma_bounds(x, 0.2, 0.01, 1.0)
+        xo = x.copy()
+        M.update(x, grad(x), g, dg, lo, hi)
+        assert np.all(x >= lo - 1e-15) and np.all(x <= hi + 1e-15)
+        assert np.abs(x - xo).max() <= 0.2 + 1e-12
+    assert abs(x.mean() - 0.4) < 1e-6
