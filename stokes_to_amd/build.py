@@ -12,6 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libstokes_hip.so")
+DRIVER = os.path.join(HERE, "fluid_topo")
 SOURCES = ["st_kernels.hip", "st_march.hip", "st_solver.hip", "st_net.hip", "st_mma.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("ST_OFFLOAD_ARCH", "gfx950")
@@ -60,7 +61,19 @@ def build(jobs=4, verbose=False):
     tmp = LIB + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
     os.replace(tmp, LIB)
+    build_driver()
     return LIB
+
+
+def build_driver():
+    """The fluid_topo command-line driver: host C++ on the C ABI only (g++), linked
+    against the library next to it."""
+    src = os.path.join(HERE, "driver", "fluid_topo.cpp")
+    tmp = DRIVER + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-Wall", src, "-o", tmp, f"-L{HERE}", "-l:libstokes_hip.so",
+          "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, DRIVER)
+    return DRIVER
 
 
 if __name__ == "__main__":

@@ -10,6 +10,26 @@ struct AlphaParams {
     double amax, amin, q, power, scale;
 };
 
+// the V-cycle below and including level l0 in one launch (levels small enough for
+// one workgroup): pre-smoothing, residual and restriction down to the coarsest
+// level, its dense solve, then prolongation and post-smoothing back to l0
+constexpr int ST_MAX_LEVELS = 12;
+constexpr long ST_TAIL_MAX_CELLS = 4096;
+// (kept in device memory: the kernel indexes it by level; b and x of level 0 are
+// the caller's vectors when l0 = 0, the level buffers b[l], x[l] otherwise)
+template <typename TS> struct TailArgs {
+    int l0, L, nu_pre, nu_post, nd, dim;
+    float omega;
+    Lvl g[ST_MAX_LEVELS];
+    const uint8_t *fl[ST_MAX_LEVELS];
+    const TS *al[ST_MAX_LEVELS];
+    TS *b[ST_MAX_LEVELS], *x[ST_MAX_LEVELS], *t[ST_MAX_LEVELS], *r[ST_MAX_LEVELS];
+    const long *dofs;
+    const double *minv;
+};
+template <typename TS>
+void launch_vcycle_tail(const TailArgs<TS> *dev_args, int dim, const TS *b0, TS *x0, int nd, hipStream_t);
+
 template <typename TS> void launch_apply(const Lvl &, const uint8_t *, const TS *, const TS *, TS *, hipStream_t);
 template <typename TS>
 void launch_residual(const Lvl &, const uint8_t *, const TS *, const TS *b, const TS *x, TS *r, double *part,
@@ -55,10 +75,10 @@ void launch_absdiff_max(long n, const double *a, const double *b, double *part, 
 void launch_sum(int np, int nvec, const double *part, double *out, hipStream_t);
 unsigned blas_blocks(long n);
 template <typename TA, typename TB> void launch_dot(long n, const TA *a, const TB *b, double *part, hipStream_t);
-template <typename TS>
-void launch_multidot(long n, const TS *w, const TS *V, long ld, int m, double *part, hipStream_t);
-template <typename TS>
-void launch_multiaxpy(long n, TS *w, const TS *V, long ld, int m, const double *h, double *part, hipStream_t);
+template <typename TW, typename TV>
+void launch_multidot(long n, const TW *w, const TV *V, long ld, int m, double *part, hipStream_t);
+template <typename TW, typename TV>
+void launch_multiaxpy(long n, TW *w, const TV *V, long ld, int m, const double *h, double *part, hipStream_t);
 template <typename TX, typename TY> void launch_axpby(long n, double a, const TX *x, double b, TY *y, hipStream_t);
 template <typename TS> void launch_fill(long n, TS *y, double v, hipStream_t);
 
@@ -72,5 +92,7 @@ void march_residual(const Lvl &, const uint8_t *, const TS *, const TS *b, const
 template <typename TS>
 void march_smooth(const Lvl &, const uint8_t *, const TS *, const TS *b, const TS *xin, TS *xout, int color,
                   float omega, hipStream_t);
+// FP64 operator applied to an FP32-stored vector (x float, alpha and y double)
+void march_apply_mixed(const Lvl &, const uint8_t *, const double *al, const float *x, double *y, hipStream_t);
 
 }  // namespace st

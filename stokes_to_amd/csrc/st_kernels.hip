@@ -158,32 +158,38 @@ __global__ void __launch_bounds__(256) k_apply(Lvl g, const uint8_t *fl, const T
     if (DIM == 3) y[3 * g.S + n] = Cvt<TS>::st(s.template Aface<2>(i, j, k));
 }
 
+// r = b - A x at node n; returns the node's contribution to |r|^2
+template <typename TS, int DIM>
+__device__ __forceinline__ double residual_body(const Lvl &g, const uint8_t *fl, const TS *al, const TS *b,
+                                                const TS *x, TS *r, long n)
+{
+    if (n >= g.S) return 0.0;
+    int i, j, k;
+    node_ijk(g, n, i, j, k);
+    Sten<TS, DIM> s{g, fl, x, al, -1};
+    typedef typename Cvt<TS>::C C;
+    C v0 = Cvt<TS>::ld(b[n]) - s.Acell(i, j, k);
+    C v1 = Cvt<TS>::ld(b[g.S + n]) - s.template Aface<0>(i, j, k);
+    C v2 = Cvt<TS>::ld(b[2 * g.S + n]) - s.template Aface<1>(i, j, k);
+    r[n] = Cvt<TS>::st(v0);
+    r[g.S + n] = Cvt<TS>::st(v1);
+    r[2 * g.S + n] = Cvt<TS>::st(v2);
+    double acc = (double)v0 * v0 + (double)v1 * v1 + (double)v2 * v2;
+    if (DIM == 3) {
+        C v3 = Cvt<TS>::ld(b[3 * g.S + n]) - s.template Aface<2>(i, j, k);
+        r[3 * g.S + n] = Cvt<TS>::st(v3);
+        acc += (double)v3 * v3;
+    }
+    return acc;
+}
+
 // r = b - A x, optional partial sums of r^2 (FP64) per block
 template <typename TS, int DIM>
 __global__ void __launch_bounds__(256) k_residual(Lvl g, const uint8_t *fl, const TS *al, const TS *b,
                                                   const TS *x, TS *r, double *part)
 {
     __shared__ double sh[16];
-    long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    double acc = 0.0;
-    if (n < g.S) {
-        int i, j, k;
-        node_ijk(g, n, i, j, k);
-        Sten<TS, DIM> s{g, fl, x, al, -1};
-        typedef typename Cvt<TS>::C C;
-        C v0 = Cvt<TS>::ld(b[n]) - s.Acell(i, j, k);
-        C v1 = Cvt<TS>::ld(b[g.S + n]) - s.template Aface<0>(i, j, k);
-        C v2 = Cvt<TS>::ld(b[2 * g.S + n]) - s.template Aface<1>(i, j, k);
-        r[n] = Cvt<TS>::st(v0);
-        r[g.S + n] = Cvt<TS>::st(v1);
-        r[2 * g.S + n] = Cvt<TS>::st(v2);
-        acc = (double)v0 * v0 + (double)v1 * v1 + (double)v2 * v2;
-        if (DIM == 3) {
-            C v3 = Cvt<TS>::ld(b[3 * g.S + n]) - s.template Aface<2>(i, j, k);
-            r[3 * g.S + n] = Cvt<TS>::st(v3);
-            acc += (double)v3 * v3;
-        }
-    }
+    const double acc = residual_body<TS, DIM>(g, fl, al, b, x, r, (long)blockIdx.x * blockDim.x + threadIdx.x);
     if (part) {
         double t = block_sum(acc, sh);
         if (threadIdx.x == 0) part[blockIdx.x] = t;
@@ -201,11 +207,10 @@ __global__ void __launch_bounds__(256) k_residual(Lvl g, const uint8_t *fl, cons
 // form through the scalar Schur complement.  xin -> xout (ping-pong); DOFs not
 // owned by a cell of the colour are copied.  ZERO: xin is identically zero.
 template <typename TS, int DIM, bool ZERO>
-__global__ void __launch_bounds__(256) k_smooth(Lvl g, const uint8_t *fl, const TS *al, const TS *b,
-                                                const TS *xin, TS *xout, int color, float omega)
+__device__ __forceinline__ void smooth_body(const Lvl &g, const uint8_t *fl, const TS *al, const TS *b,
+                                            const TS *xin, TS *xout, int color, float omega, long t)
 {
     typedef typename Cvt<TS>::C C;
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     long ncell = (long)g.nx * g.ny * g.nz;
     if (t >= ncell) return;
     int k = (int)(t % g.nz);
@@ -286,6 +291,13 @@ __global__ void __launch_bounds__(256) k_smooth(Lvl g, const uint8_t *fl, const 
     }
 }
 
+template <typename TS, int DIM, bool ZERO>
+__global__ void __launch_bounds__(256) k_smooth(Lvl g, const uint8_t *fl, const TS *al, const TS *b,
+                                                const TS *xin, TS *xout, int color, float omega)
+{
+    smooth_body<TS, DIM, ZERO>(g, fl, al, b, xin, xout, color, omega, (long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
 // ------------------------------------------------------------- transfers
 
 // coarse rhs from the fine residual: R = (4/2^dim) P^T on velocity, (2/2^dim)
@@ -294,10 +306,10 @@ __global__ void __launch_bounds__(256) k_smooth(Lvl g, const uint8_t *fl, const 
 // constant.  The reference restricts by plain 8-child sums
 // (ST/StokeFlowSubSystem3D.cu:32-52, CX/SubsystemUtils3D.cuh:7-185).
 template <typename TS, int DIM>
-__global__ void __launch_bounds__(256) k_restrict(Lvl gf, Lvl gc, const uint8_t *flc, const TS *rf, TS *bc)
+__device__ __forceinline__ void restrict_body(const Lvl &gf, const Lvl &gc, const uint8_t *flc, const TS *rf, TS *bc,
+                                              long n)
 {
     typedef typename Cvt<TS>::C C;
-    long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= gc.S) return;
     int I, J, K;
     node_ijk(gc, n, I, J, K);
@@ -335,12 +347,18 @@ __global__ void __launch_bounds__(256) k_restrict(Lvl gf, Lvl gc, const uint8_t 
     }
 }
 
+template <typename TS, int DIM>
+__global__ void __launch_bounds__(256) k_restrict(Lvl gf, Lvl gc, const uint8_t *flc, const TS *rf, TS *bc)
+{
+    restrict_body<TS, DIM>(gf, gc, flc, rf, bc, (long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
 // xf += P ec on free fine DOFs (pressure correction scaled by 1/2: unit spacing)
 template <typename TS, int DIM>
-__global__ void __launch_bounds__(256) k_prolong(Lvl gf, Lvl gc, const uint8_t *flf, const TS *ec, TS *xf)
+__device__ __forceinline__ void prolong_body(const Lvl &gf, const Lvl &gc, const uint8_t *flf, const TS *ec, TS *xf,
+                                             long n)
 {
     typedef typename Cvt<TS>::C C;
-    long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= gf.S) return;
     int i, j, k;
     node_ijk(gf, n, i, j, k);
@@ -364,6 +382,12 @@ __global__ void __launch_bounds__(256) k_prolong(Lvl gf, Lvl gc, const uint8_t *
         long o = (long)(1 + d) * gf.S + n;
         xf[o] = Cvt<TS>::st(Cvt<TS>::ld(xf[o]) + e);
     }
+}
+
+template <typename TS, int DIM>
+__global__ void __launch_bounds__(256) k_prolong(Lvl gf, Lvl gc, const uint8_t *flf, const TS *ec, TS *xf)
+{
+    prolong_body<TS, DIM>(gf, gc, flf, ec, xf, (long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // coarse flags (free if any coincident child is free) and coarse face alpha
@@ -529,6 +553,75 @@ __global__ void k_coarse_solve(int nd, const long *dofs, const double *Minv, con
         const double *row = Minv + (long)r * nd;
         for (int c = 0; c < nd; c++) s += row[c] * bs[c];
         x[dofs[r]] = Cvt<TS>::st((typename Cvt<TS>::C)s);
+    }
+}
+
+// ------------------------------------------------ fused coarse V-cycle
+
+// One workgroup runs every pass of the coarse levels: each pass is a
+// workgroup-strided loop over the level's cells or nodes, passes are separated by
+// barriers (the level data, a few hundred KB, stays in L2).  The same bodies as the
+// per-level kernels, so the result is identical; it replaces about eleven
+// latency-bound launches per level.  b[l0] is the caller's right-hand side; bw[l]
+// (l > l0) receive the restricted residuals.
+template <typename TS, int DIM>
+__global__ void __launch_bounds__(512) k_vcycle_tail(const TailArgs<TS> *__restrict__ a, const TS *b0, TS *x0)
+{
+    extern __shared__ double bs[];
+    const long T = blockDim.x;
+    const int l0 = a->l0, L = a->L;
+    const float om = a->omega;
+    auto bof = [&](int l) -> const TS * { return l == 0 ? b0 : a->b[l]; };
+    auto xof = [&](int l) -> TS * { return l == 0 ? x0 : a->x[l]; };
+    auto smooth = [&](int l, int nu, int first, bool zero) {
+        const Lvl g = a->g[l];
+        const uint8_t *fl = a->fl[l];
+        const TS *al = a->al[l], *b = bof(l);
+        TS *x = xof(l), *t = a->t[l];
+        const long nc = (long)g.nx * g.ny * g.nz;
+        for (int sw = 0; sw < nu; sw++) {
+            if (zero && sw == 0)
+                for (long q = threadIdx.x; q < nc; q += T) smooth_body<TS, DIM, true>(g, fl, al, b, x, t, first, om, q);
+            else
+                for (long q = threadIdx.x; q < nc; q += T) smooth_body<TS, DIM, false>(g, fl, al, b, x, t, first, om, q);
+            __syncthreads();
+            for (long q = threadIdx.x; q < nc; q += T) smooth_body<TS, DIM, false>(g, fl, al, b, t, x, first ^ 1, om, q);
+            __syncthreads();
+        }
+    };
+    for (int l = l0; l + 1 < L; l++) {
+        const Lvl g = a->g[l];
+        if (a->nu_pre > 0) {
+            smooth(l, a->nu_pre, 0, true);
+        } else {
+            TS *x = xof(l);
+            for (long q = threadIdx.x; q < (long)g.nfield * g.S; q += T) x[q] = Cvt<TS>::st((typename Cvt<TS>::C)0);
+            __syncthreads();
+        }
+        for (long n = threadIdx.x; n < g.S; n += T) residual_body<TS, DIM>(g, a->fl[l], a->al[l], bof(l), xof(l), a->r[l], n);
+        __syncthreads();
+        const Lvl gc = a->g[l + 1];
+        for (long n = threadIdx.x; n < gc.S; n += T) restrict_body<TS, DIM>(g, gc, a->fl[l + 1], a->r[l], a->b[l + 1], n);
+        __syncthreads();
+    }
+    const int Lc = L - 1, nd = a->nd;
+    const TS *bc = bof(Lc);
+    TS *xc = xof(Lc);
+    const long *dofs = a->dofs;
+    for (int c = threadIdx.x; c < nd; c += T) bs[c] = (double)Cvt<TS>::ld(bc[dofs[c]]);
+    __syncthreads();
+    for (int r = threadIdx.x; r < nd; r += T) {
+        double sacc = 0.0;
+        const double *row = a->minv + (long)r * nd;
+        for (int c = 0; c < nd; c++) sacc += row[c] * bs[c];
+        xc[dofs[r]] = Cvt<TS>::st((typename Cvt<TS>::C)sacc);
+    }
+    __syncthreads();
+    for (int l = L - 2; l >= l0; l--) {
+        const Lvl g = a->g[l];
+        for (long n = threadIdx.x; n < g.S; n += T) prolong_body<TS, DIM>(g, a->g[l + 1], a->fl[l], a->x[l + 1], xof(l), n);
+        __syncthreads();
+        if (a->nu_post > 0) smooth(l, a->nu_post, 1, false);
     }
 }
 
@@ -837,18 +930,20 @@ __global__ void __launch_bounds__(256) k_dot(long n, const TA *a, const TB *b, d
 }
 
 // dots of w with the m vectors V[0..m), V contiguous with stride ld
-template <typename TS, int M>
-__global__ void __launch_bounds__(256) k_multidot(long n, const TS *w, const TS *V, long ld, int m, double *part)
+// partial sums of w . V[v] for v < m (FP64 accumulation; V may be stored in a
+// lower precision than w)
+template <typename TW, typename TV, int M>
+__global__ void __launch_bounds__(256) k_multidot(long n, const TW *w, const TV *V, long ld, int m, double *part)
 {
     __shared__ double sh[16];
     double acc[M];
 #pragma unroll
     for (int v = 0; v < M; v++) acc[v] = 0.0;
     for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
-        const double wv = (double)Cvt<TS>::ld(w[q]);
+        const double wv = (double)Cvt<TW>::ld(w[q]);
 #pragma unroll
         for (int v = 0; v < M; v++)
-            if (v < m) acc[v] += wv * (double)Cvt<TS>::ld(V[(long)v * ld + q]);
+            if (v < m) acc[v] += wv * (double)Cvt<TV>::ld(V[(long)v * ld + q]);
     }
     for (int v = 0; v < m; v++) {
         double s = block_sum(acc[v], sh);
@@ -857,8 +952,8 @@ __global__ void __launch_bounds__(256) k_multidot(long n, const TS *w, const TS 
 }
 
 // w -= sum_v h[v] V[v]; partial sums of |w|^2
-template <typename TS, int M>
-__global__ void __launch_bounds__(256) k_multiaxpy(long n, TS *w, const TS *V, long ld, int m, const double *h,
+template <typename TW, typename TV, int M>
+__global__ void __launch_bounds__(256) k_multiaxpy(long n, TW *w, const TV *V, long ld, int m, const double *h,
                                                    double *part)
 {
     __shared__ double sh[16];
@@ -867,11 +962,11 @@ __global__ void __launch_bounds__(256) k_multiaxpy(long n, TS *w, const TS *V, l
     __syncthreads();
     double acc = 0.0;
     for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
-        double wv = (double)Cvt<TS>::ld(w[q]);
+        double wv = (double)Cvt<TW>::ld(w[q]);
 #pragma unroll
         for (int v = 0; v < M; v++)
-            if (v < m) wv -= hs[v] * (double)Cvt<TS>::ld(V[(long)v * ld + q]);
-        w[q] = Cvt<TS>::st((typename Cvt<TS>::C)wv);
+            if (v < m) wv -= hs[v] * (double)Cvt<TV>::ld(V[(long)v * ld + q]);
+        w[q] = Cvt<TW>::st((typename Cvt<TW>::C)wv);
         acc += wv * wv;
     }
     if (part) {
@@ -1057,19 +1152,19 @@ void launch_dot(long n, const TA *a, const TB *b, double *part, hipStream_t s)
 {
     hipLaunchKernelGGL((k_dot<TA, TB>), dim3(nb_grid(n)), dim3(256), 0, s, n, a, b, part);
 }
-template <typename TS>
-void launch_multidot(long n, const TS *w, const TS *V, long ld, int m, double *part, hipStream_t s)
+template <typename TW, typename TV>
+void launch_multidot(long n, const TW *w, const TV *V, long ld, int m, double *part, hipStream_t s)
 {
-    if (m <= 8) hipLaunchKernelGGL((k_multidot<TS, 8>), dim3(nb_grid(n)), dim3(256), 0, s, n, w, V, ld, m, part);
-    else if (m <= 16) hipLaunchKernelGGL((k_multidot<TS, 16>), dim3(nb_grid(n)), dim3(256), 0, s, n, w, V, ld, m, part);
-    else hipLaunchKernelGGL((k_multidot<TS, 32>), dim3(nb_grid(n)), dim3(256), 0, s, n, w, V, ld, m, part);
+    if (m <= 8) hipLaunchKernelGGL((k_multidot<TW, TV, 8>), dim3(nb_grid(n)), dim3(256), 0, s, n, w, V, ld, m, part);
+    else if (m <= 16) hipLaunchKernelGGL((k_multidot<TW, TV, 16>), dim3(nb_grid(n)), dim3(256), 0, s, n, w, V, ld, m, part);
+    else hipLaunchKernelGGL((k_multidot<TW, TV, 32>), dim3(nb_grid(n)), dim3(256), 0, s, n, w, V, ld, m, part);
 }
-template <typename TS>
-void launch_multiaxpy(long n, TS *w, const TS *V, long ld, int m, const double *h, double *part, hipStream_t s)
+template <typename TW, typename TV>
+void launch_multiaxpy(long n, TW *w, const TV *V, long ld, int m, const double *h, double *part, hipStream_t s)
 {
-    if (m <= 8) hipLaunchKernelGGL((k_multiaxpy<TS, 8>), dim3(nb_grid(n)), dim3(256), 0, s, n, w, V, ld, m, h, part);
-    else if (m <= 16) hipLaunchKernelGGL((k_multiaxpy<TS, 16>), dim3(nb_grid(n)), dim3(256), 0, s, n, w, V, ld, m, h, part);
-    else hipLaunchKernelGGL((k_multiaxpy<TS, 32>), dim3(nb_grid(n)), dim3(256), 0, s, n, w, V, ld, m, h, part);
+    if (m <= 8) hipLaunchKernelGGL((k_multiaxpy<TW, TV, 8>), dim3(nb_grid(n)), dim3(256), 0, s, n, w, V, ld, m, h, part);
+    else if (m <= 16) hipLaunchKernelGGL((k_multiaxpy<TW, TV, 16>), dim3(nb_grid(n)), dim3(256), 0, s, n, w, V, ld, m, h, part);
+    else hipLaunchKernelGGL((k_multiaxpy<TW, TV, 32>), dim3(nb_grid(n)), dim3(256), 0, s, n, w, V, ld, m, h, part);
 }
 template <typename TX, typename TY>
 void launch_axpby(long n, double a, const TX *x, double b, TY *y, hipStream_t s)
@@ -1080,6 +1175,14 @@ template <typename TS>
 void launch_fill(long n, TS *y, double v, hipStream_t s)
 {
     hipLaunchKernelGGL((k_fill<TS>), dim3(nb_grid(n)), dim3(256), 0, s, n, y, v);
+}
+
+template <typename TS>
+void launch_vcycle_tail(const TailArgs<TS> *a, int dim, const TS *b0, TS *x0, int nd, hipStream_t s)
+{
+    const size_t sh = sizeof(double) * (size_t)(nd > 0 ? nd : 1);
+    if (dim == 3) hipLaunchKernelGGL((k_vcycle_tail<TS, 3>), dim3(1), dim3(512), sh, s, a, b0, x0);
+    else hipLaunchKernelGGL((k_vcycle_tail<TS, 2>), dim3(1), dim3(512), sh, s, a, b0, x0);
 }
 
 // explicit instantiations
@@ -1097,9 +1200,8 @@ void launch_fill(long n, TS *y, double v, hipStream_t s)
     template void launch_pack<T>(const Lvl &, const double *, T *, int, const uint8_t *, double, hipStream_t); \
     template void launch_unpack<T>(const Lvl &, const T *, double *, hipStream_t);                          \
     template void launch_alpha_from_pen<T>(const Lvl &, const double *, const uint8_t *, T *, hipStream_t);  \
-    template void launch_multidot<T>(long, const T *, const T *, long, int, double *, hipStream_t);          \
-    template void launch_multiaxpy<T>(long, T *, const T *, long, int, const double *, double *, hipStream_t); \
-    template void launch_fill<T>(long, T *, double, hipStream_t);
+    template void launch_fill<T>(long, T *, double, hipStream_t);                                           \
+    template void launch_vcycle_tail<T>(const TailArgs<T> *, int, const T *, T *, int, hipStream_t);
 
 INST_TS(double)
 INST_TS(float)
@@ -1118,6 +1220,12 @@ INST_PAIR(double, bf16)
 INST_PAIR(bf16, double)
 INST_PAIR(bf16, bf16)
 INST_PAIR(float, bf16)
+
+#define INST_KRYLOV(W, V)                                                                                   \
+    template void launch_multidot<W, V>(long, const W *, const V *, long, int, double *, hipStream_t);        \
+    template void launch_multiaxpy<W, V>(long, W *, const V *, long, int, const double *, double *, hipStream_t);
+INST_KRYLOV(double, double)
+INST_KRYLOV(double, float)
 INST_PAIR(bf16, float)
 
 }  // namespace st

@@ -33,9 +33,11 @@ __device__ __forceinline__ double wsum64(double v)
 }
 }  // namespace
 
-template <typename TS, int MODE>
+// TS: storage of alpha, b and y (and the compute type through Cvt<TS>::C); TX: storage
+// of x (a lower-precision x is widened on load: the FP64 operator on an FP32 vector)
+template <typename TS, int MODE, typename TX = TS>
 __global__ void __launch_bounds__(256) k_march(Lvl g, const uint8_t *__restrict__ fl, const TS *__restrict__ al,
-                                               const TS *__restrict__ b, const TS *__restrict__ x,
+                                               const TS *__restrict__ b, const TX *__restrict__ x,
                                                TS *__restrict__ y, double *part, int chunk, int color, float omega)
 {
     typedef typename Cvt<TS>::C C;
@@ -80,10 +82,10 @@ __global__ void __launch_bounds__(256) k_march(Lvl g, const uint8_t *__restrict_
             if (t < NL && s >= 0 && s <= g.nx && jj >= 0 && jj <= g.ny && kk >= 0 && kk <= g.nz) {
                 const long q = (long)s * sI + (long)jj * sJ + kk;
                 rf[e] = fl[q];
-                rw[e][0] = Cvt<TS>::ld(x[q]);
-                rw[e][1] = Cvt<TS>::ld(x[S + q]);
-                rw[e][2] = Cvt<TS>::ld(x[2 * S + q]);
-                rw[e][3] = Cvt<TS>::ld(x[3 * S + q]);
+                rw[e][0] = (C)Cvt<TX>::ld(x[q]);
+                rw[e][1] = (C)Cvt<TX>::ld(x[S + q]);
+                rw[e][2] = (C)Cvt<TX>::ld(x[2 * S + q]);
+                rw[e][3] = (C)Cvt<TX>::ld(x[3 * S + q]);
             }
         }
     };
@@ -160,9 +162,9 @@ __global__ void __launch_bounds__(256) k_march(Lvl g, const uint8_t *__restrict_
             pb[2] = Cvt<TS>::ld(b[2 * S + q]);
             pb[3] = Cvt<TS>::ld(b[3 * S + q]);
         }
-        px[0] = Cvt<TS>::ld(x[S + q]);
-        px[1] = Cvt<TS>::ld(x[2 * S + q]);
-        px[2] = Cvt<TS>::ld(x[3 * S + q]);
+        px[0] = (C)Cvt<TX>::ld(x[S + q]);
+        px[1] = (C)Cvt<TX>::ld(x[2 * S + q]);
+        px[2] = (C)Cvt<TX>::ld(x[3 * S + q]);
         pa[0] = Cvt<TS>::ld(al[q]);
         pa[1] = Cvt<TS>::ld(al[S + q]);
         pa[2] = Cvt<TS>::ld(al[2 * S + q]);
@@ -170,9 +172,9 @@ __global__ void __launch_bounds__(256) k_march(Lvl g, const uint8_t *__restrict_
             pb[4] = Cvt<TS>::ld(b[S + q + sI]);
             pb[5] = Cvt<TS>::ld(b[2 * S + q + sJ]);
             pb[6] = Cvt<TS>::ld(b[3 * S + q + 1]);
-            px[3] = Cvt<TS>::ld(x[S + q + sI]);
-            px[4] = Cvt<TS>::ld(x[2 * S + q + sJ]);
-            px[5] = Cvt<TS>::ld(x[3 * S + q + 1]);
+            px[3] = (C)Cvt<TX>::ld(x[S + q + sI]);
+            px[4] = (C)Cvt<TX>::ld(x[2 * S + q + sJ]);
+            px[5] = (C)Cvt<TX>::ld(x[3 * S + q + 1]);
             pa[3] = Cvt<TS>::ld(al[q + sI]);
             pa[4] = Cvt<TS>::ld(al[S + q + sJ]);
             pa[5] = Cvt<TS>::ld(al[2 * S + q + 1]);
@@ -234,7 +236,7 @@ __global__ void __launch_bounds__(256) k_march(Lvl g, const uint8_t *__restrict_
                 for (int f = 0; f < 4; f++) bb[f] = (MODE == M_RESID) ? Cvt<TS>::ld(b[(long)f * S + qq]) : C(0);
 #pragma unroll
                 for (int f = 0; f < 3; f++) {
-                    xr[f] = Cvt<TS>::ld(x[(long)(f + 1) * S + qq]);
+                    xr[f] = (C)Cvt<TX>::ld(x[(long)(f + 1) * S + qq]);
                     av[f] = Cvt<TS>::ld(al[(long)f * S + qq]);
                 }
                 emit(c, qq, C(0), C(0), bb, xr, av);
@@ -378,6 +380,14 @@ void march_smooth(const Lvl &g, const uint8_t *fl, const TS *al, const TS *b, co
     dim3 grid = march_grid(g, ch);
     hipLaunchKernelGGL((k_march<TS, M_SMOOTH>), grid, dim3(NT), 0, s, g, fl, al, b, xin, xout, (double *)nullptr,
                        ch, color, omega);
+}
+
+void march_apply_mixed(const Lvl &g, const uint8_t *fl, const double *al, const float *x, double *y, hipStream_t s)
+{
+    int ch;
+    dim3 grid = march_grid(g, ch);
+    hipLaunchKernelGGL((k_march<double, M_APPLY, float>), grid, dim3(NT), 0, s, g, fl, al, (const double *)nullptr, x,
+                       y, (double *)nullptr, ch, 0, 0.f);
 }
 
 #define INST_M(T)                                                                                              \

@@ -28,6 +28,7 @@ namespace {
 
 template <typename TS> struct Hier {
     std::vector<TS *> al, b, x, t, r;
+    TailArgs<TS> *tail = nullptr;  // device copy of the fused coarse V-cycle's arguments
 };
 
 struct Net;  // learned predictor (st_net.hip)
@@ -251,11 +252,23 @@ template <typename TS> static void smooth(st_ctx *c, int l, const TS *b, TS *x, 
     }
 }
 
+// first level handled by the fused coarse V-cycle: the largest level with at most
+// ST_TAIL_MAX_CELLS cells (one workgroup), or L when there is none
+static int tail_level(const st_ctx *c)
+{
+    for (int l = 0; l < c->L; l++) {
+        const Lvl &g = c->lv[l];
+        if ((long)g.nx * g.ny * g.nz <= ST_TAIL_MAX_CELLS) return l;
+    }
+    return c->L;
+}
+
 template <typename TS> static void vcycle(st_ctx *c, const TS *b0, TS *x0)
 {
     Hier<TS> &h = hier<TS>(c);
     const int L = c->L;
-    for (int l = 0; l + 1 < L; l++) {
+    const int lt = tail_level(c);
+    for (int l = 0; l + 1 < L && l < lt; l++) {
         const Lvl &g = c->lv[l];
         const TS *b = (l == 0) ? b0 : h.b[l];
         TS *x = (l == 0) ? x0 : h.x[l];
@@ -266,8 +279,59 @@ template <typename TS> static void vcycle(st_ctx *c, const TS *b0, TS *x0)
         if (l == 0) prof_mark(c, ST_PROF_RESIDUAL);
         launch_restrict<TS>(g, c->lv[l + 1], c->fl[l + 1], h.r[l], h.b[l + 1], c->s);
     }
-    launch_coarse_solve<TS>(c->nd, c->d_dofs, c->d_minv, h.b[L - 1], h.x[L - 1], c->s);
-    for (int l = L - 2; l >= 0; l--) {
+    if (lt < L) {
+        if (!h.tail) {  // built once: the level buffers do not move
+            TailArgs<TS> a;
+            std::memset(&a, 0, sizeof(a));
+            a.l0 = lt;
+            a.L = L;
+            a.nu_pre = c->prm.nu_pre;
+            a.nu_post = c->prm.nu_post;
+            a.nd = c->nd;
+            a.dim = c->dim;
+            a.omega = (float)c->prm.omega;
+            for (int l = lt; l < L; l++) {
+                a.g[l] = c->lv[l];
+                a.fl[l] = c->fl[l];
+                a.al[l] = h.al[l];
+                a.b[l] = h.b[l];
+                a.x[l] = h.x[l];
+                a.t[l] = h.t[l];
+                a.r[l] = h.r[l];
+            }
+            a.dofs = c->d_dofs;
+            a.minv = c->d_minv;
+            void *p = nullptr;
+            if (hipMalloc(&p, sizeof(a)) == hipSuccess && hipMemcpy(p, &a, sizeof(a), hipMemcpyHostToDevice) == hipSuccess) {
+                c->allocs.push_back(p);
+                h.tail = (TailArgs<TS> *)p;
+            }
+        }
+        if (h.tail) {
+            launch_vcycle_tail<TS>(h.tail, c->dim, lt == 0 ? b0 : h.b[lt], lt == 0 ? x0 : h.x[lt], c->nd, c->s);
+        } else {  // allocation failed: per-level kernels (same arithmetic)
+            for (int l = lt; l + 1 < L; l++) {
+                const Lvl &g = c->lv[l];
+                const TS *b = (l == 0) ? b0 : h.b[l];
+                TS *x = (l == 0) ? x0 : h.x[l];
+                if (c->prm.nu_pre > 0) smooth<TS>(c, l, b, x, c->prm.nu_pre, 0, true);
+                else launch_fill<TS>((long)g.nfield * g.S, x, 0.0, c->s);
+                launch_residual<TS>(g, c->fl[l], h.al[l], b, x, h.r[l], nullptr, c->s);
+                launch_restrict<TS>(g, c->lv[l + 1], c->fl[l + 1], h.r[l], h.b[l + 1], c->s);
+            }
+            launch_coarse_solve<TS>(c->nd, c->d_dofs, c->d_minv, (L - 1 == 0) ? b0 : h.b[L - 1],
+                                    (L - 1 == 0) ? x0 : h.x[L - 1], c->s);
+            for (int l = L - 2; l >= lt; l--) {
+                const TS *b = (l == 0) ? b0 : h.b[l];
+                TS *x = (l == 0) ? x0 : h.x[l];
+                launch_prolong<TS>(c->lv[l], c->lv[l + 1], c->fl[l], h.x[l + 1], x, c->s);
+                if (c->prm.nu_post > 0) smooth<TS>(c, l, b, x, c->prm.nu_post, 1, false);
+            }
+        }
+    } else {
+        launch_coarse_solve<TS>(c->nd, c->d_dofs, c->d_minv, h.b[L - 1], h.x[L - 1], c->s);
+    }
+    for (int l = std::min(lt, L - 1) - 1; l >= 0; l--) {
         const TS *b = (l == 0) ? b0 : h.b[l];
         TS *x = (l == 0) ? x0 : h.x[l];
         launch_prolong<TS>(c->lv[l], c->lv[l + 1], c->fl[l], h.x[l + 1], x, c->s);
@@ -275,28 +339,48 @@ template <typename TS> static void vcycle(st_ctx *c, const TS *b0, TS *x0)
     }
 }
 
-// z = M v for FP64 Krylov vectors
-static void precondition(st_ctx *c, const double *v, double *z)
+// z = M v.  The FP64 path keeps FP64 Krylov vectors; the mixed-precision paths
+// keep them in FP32 (the V-cycle works on them directly in FP32, through a BF16
+// copy in the BF16 path)
+static void precondition(st_ctx *c, const double *v, double *z) { vcycle<double>(c, v, z); }
+static void precondition(st_ctx *c, const float *v, float *z)
 {
-    const long nv = c->nvec;
-    if (c->prm.precision == ST_FP64) {
-        vcycle<double>(c, v, z);
-    } else if (c->prm.precision == ST_FP32) {
-        launch_axpby<double, float>(nv, 1.0, v, 0.0, c->h32.b[0], c->s);
-        vcycle<float>(c, c->h32.b[0], c->h32.x[0]);
-        launch_axpby<float, double>(nv, 1.0, c->h32.x[0], 0.0, z, c->s);
+    if (c->prm.precision == ST_FP32) {
+        vcycle<float>(c, v, z);
     } else {
-        launch_axpby<double, bf16>(nv, 1.0, v, 0.0, c->hbf.b[0], c->s);
+        const long nv = c->nvec;
+        launch_axpby<float, bf16>(nv, 1.0, v, 0.0, c->hbf.b[0], c->s);
         vcycle<bf16>(c, c->hbf.b[0], c->hbf.x[0]);
-        launch_axpby<bf16, double>(nv, 1.0, c->hbf.x[0], 0.0, z, c->s);
+        launch_axpby<bf16, float>(nv, 1.0, c->hbf.x[0], 0.0, z, c->s);
+    }
+}
+
+// w = A z with the FP64 operator
+static void apply64(st_ctx *c, const double *z, double *w)
+{
+    launch_apply<double>(c->lv[0], c->fl[0], c->h64.al[0], z, w, c->s);
+}
+static void apply64(st_ctx *c, const float *z, double *w)
+{
+    const Lvl &g = c->lv[0];
+    if (march_ok(g)) {
+        march_apply_mixed(g, c->fl[0], c->h64.al[0], z, w, c->s);
+    } else {
+        launch_axpby<float, double>(c->nvec, 1.0, z, 0.0, c->h64.t[0], c->s);
+        launch_apply<double>(g, c->fl[0], c->h64.al[0], c->h64.t[0], w, c->s);
     }
 }
 
 // ----------------------------------------------------------------- FGMRES
 
-// solves A x = b (padded FP64, device), x holds the initial guess on entry
-static int fgmres(st_ctx *c, const double *b, double *x, st_stats *st)
+// solves A x = b (padded FP64, device), x holds the initial guess on entry.
+// TV: storage of the Krylov vectors V and Z (FP64 or FP32); x, w, the inner products
+// and the stopping residual are FP64 (the true residual is recomputed at every
+// restart, which bounds the effect of FP32 storage to one restart cycle)
+template <typename TV> static int fgmres(st_ctx *c, const double *b, double *x, st_stats *st)
 {
+    TV *const Vb = reinterpret_cast<TV *>(c->V);
+    TV *const Zs = reinterpret_cast<TV *>(c->Zb);
     const Lvl &g = c->lv[0];
     const long nv = c->nvec;
     const int m = c->prm.restart;
@@ -311,8 +395,8 @@ static int fgmres(st_ctx *c, const double *b, double *x, st_stats *st)
     double rr = 0.0, rr0 = -1.0;
     const int rb = (int)residual_blocks(g);
     for (;;) {
-        double *v0 = c->V;
-        launch_residual<double>(g, c->fl[0], c->h64.al[0], b, x, v0, c->part, c->s);
+        double *r0 = std::is_same<TV, double>::value ? c->V : c->Wv;
+        launch_residual<double>(g, c->fl[0], c->h64.al[0], b, x, r0, c->part, c->s);
         launch_sum(rb, 1, c->part, c->red, c->s);
         double r2 = 0.0;
         CK(hipMemcpyAsync(&r2, c->red, sizeof(double), hipMemcpyDeviceToHost, c->s));
@@ -322,20 +406,20 @@ static int fgmres(st_ctx *c, const double *b, double *x, st_stats *st)
         rr = beta / bn;
         if (rr0 < 0) rr0 = rr;
         if (rr <= rtol || it >= c->prm.max_iter) break;
-        launch_axpby<double, double>(nv, 1.0 / beta, v0, 0.0, v0, c->s);
+        launch_axpby<double, TV>(nv, 1.0 / beta, r0, 0.0, Vb, c->s);
         std::fill(gv.begin(), gv.end(), 0.0);
         gv[0] = beta;
         int j = 0;
         for (; j < m && it < c->prm.max_iter; j++) {
-            double *vj = c->V + (long)j * nv, *zj = c->Zb + (long)j * nv;
+            TV *vj = Vb + (long)j * nv, *zj = Zs + (long)j * nv;
             precondition(c, vj, zj);
             prof_mark(c, ST_PROF_APPLY);
-            launch_apply<double>(g, c->fl[0], c->h64.al[0], zj, c->Wv, c->s);
+            apply64(c, zj, c->Wv);
             prof_mark(c, ST_PROF_APPLY);
             const unsigned nbk = blas_blocks(nv);
-            launch_multidot<double>(nv, c->Wv, c->V, nv, j + 1, c->part, c->s);
+            launch_multidot<double, TV>(nv, c->Wv, Vb, nv, j + 1, c->part, c->s);
             launch_sum((int)nbk, j + 1, c->part, c->h_dev, c->s);
-            launch_multiaxpy<double>(nv, c->Wv, c->V, nv, j + 1, c->h_dev, c->part, c->s);
+            launch_multiaxpy<double, TV>(nv, c->Wv, Vb, nv, j + 1, c->h_dev, c->part, c->s);
             launch_sum((int)nbk, 1, c->part, c->h_dev + (j + 1), c->s);
             CK(hipMemcpyAsync(hh.data(), c->h_dev, sizeof(double) * (j + 2), hipMemcpyDeviceToHost, c->s));
             CK(hipStreamSynchronize(c->s));
@@ -343,7 +427,7 @@ static int fgmres(st_ctx *c, const double *b, double *x, st_stats *st)
             if (!std::isfinite(hn)) return fail(c, ST_ENAN, "non-finite Arnoldi vector");
             for (int i = 0; i <= j; i++) H[(size_t)i * m + j] = hh[i];
             H[(size_t)(j + 1) * m + j] = hn;
-            if (hn > 0) launch_axpby<double, double>(nv, 1.0 / hn, c->Wv, 0.0, c->V + (long)(j + 1) * nv, c->s);
+            if (hn > 0) launch_axpby<double, TV>(nv, 1.0 / hn, c->Wv, 0.0, Vb + (long)(j + 1) * nv, c->s);
             for (int i = 0; i < j; i++) {
                 double a = H[(size_t)i * m + j], bb = H[(size_t)(i + 1) * m + j];
                 H[(size_t)i * m + j] = cs[i] * a + sn[i] * bb;
@@ -370,7 +454,7 @@ static int fgmres(st_ctx *c, const double *b, double *x, st_stats *st)
         }
         for (int i = 0; i < j; i++) y[i] = -y[i];
         CK(hipMemcpyAsync(c->h_dev, y.data(), sizeof(double) * j, hipMemcpyHostToDevice, c->s));
-        launch_multiaxpy<double>(nv, x, c->Zb, nv, j, c->h_dev, nullptr, c->s);
+        launch_multiaxpy<double, TV>(nv, x, Zs, nv, j, c->h_dev, nullptr, c->s);
     }
     if (st) {
         st->iters = it;
@@ -461,6 +545,10 @@ int st_create(st_ctx **out, const st_params *p)
         m /= 2;
     }
     c->L = (int)c->lv.size();
+    if (c->L > ST_MAX_LEVELS) {
+        st_destroy(c);
+        return ST_EINVAL;
+    }
     int rc = ST_OK;
     c->fl.assign(c->L, nullptr);
     for (int l = 0; l < c->L && rc == ST_OK; l++) rc = dalloc(c, &c->fl[l], c->lv[l].S);
@@ -476,7 +564,7 @@ int st_create(st_ctx **out, const st_params *p)
         rc = alloc_hier<double>(c, false);
         if (rc == ST_OK) c->h64.al[0] = al0;
     }
-    if (rc == ST_OK && p->precision == ST_FP32) rc = alloc_hier<float>(c, true);
+    if (rc == ST_OK && p->precision == ST_FP32) rc = alloc_hier<float>(c, false);
     if (rc == ST_OK && p->precision == ST_BF16) rc = alloc_hier<bf16>(c, true);
     if (rc == ST_OK) rc = dalloc(c, &c->X, c->nvec);
     if (rc == ST_OK) rc = dalloc(c, &c->B, c->nvec);
@@ -697,7 +785,7 @@ static int solve_core(st_ctx *c, const double *b, double *x, const double *xd, i
     if (c->dirty) CKR(setup_hierarchy(c));
     local.setup_ms = c->last_setup_ms;
     if (init == ST_INIT_ZERO) launch_axpby<double, double>(c->nvec, 1.0, xd, 0.0, x, c->s);
-    int rc = fgmres(c, b, x, &local);
+    int rc = (c->prm.precision == ST_FP64) ? fgmres<double>(c, b, x, &local) : fgmres<float>(c, b, x, &local);
     CK(hipEventRecord(c->e1, c->s));
     CK(hipEventSynchronize(c->e1));
     float ms = 0.f;
@@ -1193,7 +1281,14 @@ int st_mg_vcycle(st_ctx *c, const double *b, double *z)
     if (c->dirty) CKR(setup_hierarchy(c));
     CKR(stage_in(c, b, c->ndof));
     launch_pack<double>(c->lv[0], c->abi_a, c->BA, 0, c->fl[0], 1.0, c->s);
-    precondition(c, c->BA, c->Wv);
+    if (c->prm.precision == ST_FP64) {
+        precondition(c, c->BA, c->Wv);
+    } else {  // through the FP32 Krylov storage, as inside FGMRES
+        float *v = reinterpret_cast<float *>(c->V), *z = reinterpret_cast<float *>(c->Zb);
+        launch_axpby<double, float>(c->nvec, 1.0, c->BA, 0.0, v, c->s);
+        precondition(c, v, z);
+        launch_axpby<float, double>(c->nvec, 1.0, z, 0.0, c->Wv, c->s);
+    }
     launch_unpack<double>(c->lv[0], c->Wv, c->abi_b, c->s);
     return stage_out(c, z, c->ndof);
 }

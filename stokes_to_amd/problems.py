@@ -118,6 +118,64 @@ def channel_problem(dim, n, patch=0.25, alpha_max=1e4, q=0.1, peak=1.0):
                    closed=False, name=f"channel{dim}d_{n}")
 
 
+def reference_test0(dim, n):
+    """The reference's own demo case, test 0 (`fluid_topo -driver 0|1 -test 0`): a
+    closed box (every boundary-normal face fixed, pressure gauge on) driven by
+    source/sink terms, with the reference's material model alpha_os = 500 (32/n),
+    alpha_us = 1e-3, q = 0.014 (2D) / 0.01 (3D) on rho^3, unit spacing.
+
+    3D (FT/FluidTopoOptDriver3D.cpp:99-167): a source of 600 on the cells (0, j, k)
+    and x-faces (1, j, k), j, k in [28, 36); three sinks of 200 on the x = n-1 side
+    (j in [10, 18), k in [34, 42)), the z = n-1 side (i in [28, 36), j in [22, 30))
+    and the z = 0 side (i in [22, 30), j in [28, 36)).  The reference uses these
+    absolute indices at every size (they fit from n = 42); below n = 64 they are
+    scaled by n/64 here so that small grids keep the same layout.
+    2D (FT/FluidTopoOptDriver.cpp:104-166): inflow e = 60 on the cells (0, i) and
+    x-faces (1, i) with |i - n/2| <= n/32, and 16 outflows of strength
+    r = e (2 n/32 + 1) / (2 n/64 + 1) / 16 centred at n/32 (2o + 1) on the x = n-1 side.
+    """
+    nc, nfx, nfy, nfz, tot = dof_sizes(dim, n)
+    fixed = np.zeros(tot, np.uint8)
+    body = np.zeros(tot)
+    cf, fx, fy, *fzr = _split(dim, n, fixed)
+    cb, bx, by, *bzr = _split(dim, n, body)
+    fx[0] = 1
+    fx[n] = 1
+    fy[:, 0] = 1
+    fy[:, n] = 1
+    if dim == 3:
+        fz, bz = fzr[0], bzr[0]
+        fz[:, :, 0] = 1
+        fz[:, :, n] = 1
+        sc = (lambda v: v) if n >= 64 else (lambda v: v * n // 64)
+        r = lambda a, b: slice(sc(a), sc(b))
+        cb[0, r(28, 36), r(28, 36)] = 600.0
+        bx[1, r(28, 36), r(28, 36)] = 600.0
+        cb[n - 1, r(10, 18), r(34, 42)] = -200.0
+        bx[n - 1, r(10, 18), r(34, 42)] = 200.0
+        cb[r(28, 36), r(22, 30), n - 1] = -200.0
+        bz[r(28, 36), r(22, 30), n - 1] = 200.0
+        cb[r(22, 30), r(28, 36), 0] = -200.0
+        bz[r(22, 30), r(28, 36), 1] = -200.0
+        am = AlphaModel(alpha_max=500.0 / (n / 32.0), alpha_min=1e-3, q=0.01, power=3.0, scale=1.0)
+    else:
+        O = 16
+        hin, hout = n // 32, n // (4 * O)
+        e = 60.0
+        rr = e * (2 * hin + 1) / (2 * hout + 1) / O
+        i = np.arange(n)
+        inflow = np.abs(i - n // 2) <= hin
+        cb[0, inflow, 0] = e
+        bx[1, inflow, 0] = e
+        for o in range(O):
+            out = np.abs(i - n // (2 * O) * (2 * o + 1)) <= hout
+            cb[n - 1, out, 0] = -rr
+            bx[n - 1, out, 0] = rr
+        am = AlphaModel(alpha_max=500.0 / (n / 32.0), alpha_min=1e-3, q=0.014, power=3.0, scale=1.0)
+    return Problem(dim=dim, n=n, fixed=fixed, dirichlet=np.zeros(tot), body=body, alpha=am, closed=True,
+                   name=f"test0_{dim}d_{n}")
+
+
 def grf_density(dim, n, mean=0.3, sigma=8.0, seed=12345, rho_min=1e-3):
     """Seeded Gaussian random field: white noise N(0,1) convolved (FFT) with a
     Gaussian of std `sigma` cells, normalised to unit variance, logistic-squashed

@@ -172,10 +172,14 @@ def test_fine_level_tiled_kernels_match_oracle(make_solver, orc, dim, n):
         assert rel(s.mg_smooth(0, color, b, x), xo) < 1e-12
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "bf16"])
 @pytest.mark.parametrize("dim,n", [(3, 64)])
-def test_fine_level_solve_64(make_solver, orc, dim, n):
+def test_fine_level_solve_64(make_solver, orc, dim, n, prec):
+    """Tiled kernels in a whole solve; the mixed-precision paths keep their Krylov
+    vectors in FP32 and apply the FP64 operator to FP32 vectors (march_apply_mixed),
+    the true FP64 residual must still reach rtol."""
     P, rho = setup(dim, n)
-    s = solver_for(make_solver, P, rho, rtol=1e-8)
+    s = solver_for(make_solver, P, rho, rtol=1e-8, precision=prec)
     fixed, vol, pen = oracle_desc(orc, P, rho)
     x, st = s.solve()
     assert st["converged"] and st["iters"] <= 15
