@@ -120,7 +120,7 @@ def run_to(args, prob):
     K = new_ctx(args, prob, args.mp)
     rho0 = np.full(args.n ** 3, args.volfrac)
     L.set_design(rho0)
-    plain, learned, keep, train_ms, inf_ms, used = [], [], [], [], [], []
+    plain, learned, keep, train_ms, inf_ms, used, lossr = [], [], [], [], [], [], []
     total = args.warmup + args.steps
     for it in range(total):
         timed = it >= args.warmup
@@ -141,6 +141,8 @@ def run_to(args, prob):
             learned.append(sl)
             inf_ms.append(sl["init_ms"])
             used.append(int(sl["learned_used"]))
+            lo, lz = L.net_eval()  # untimed: the network's loss on this unseen pair vs predicting zero
+            lossr.append(lo / lz if lz > 0 else float("nan"))
         else:
             t0 = time.perf_counter()
             L.record_sample()
@@ -158,6 +160,7 @@ def run_to(args, prob):
     extra = {"train_ms_per_iter": train_ms, "train_ms_total": float(sum(train_ms)),
              "inference_ms_median": float(np.median(inf_ms)) if inf_ms else 0.0,
              "learned_taken": int(sum(used)), "prof": prof, "last_design": last_rhof,
+             "loss_vs_zero_median": float(np.median(lossr)) if lossr else None,
              "final_J": r["J"], "final_vol": r["vol"]}
     for s in (L, Z, K):
         s.close()
@@ -257,7 +260,7 @@ def main():
             "speedup_vs_plain": round(plain["median_ms"] / value, 3),
             "speedup_keep_vs_plain": round(plain["median_ms"] / keep["median_ms"], 3),
             "learned": {k: extra[k] for k in ("train_ms_total", "train_ms_per_iter", "inference_ms_median",
-                                              "learned_taken")},
+                                              "learned_taken", "loss_vs_zero_median")},
             "final_objective": extra["final_J"], "final_volume": extra["final_vol"],
             "roofline": roofline_of(extra["prof"]), "kernels_plain_fp64": extra["prof"],
         })

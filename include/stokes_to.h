@@ -113,6 +113,13 @@ int st_set_rhs(st_ctx *c, const double *body, const double *dirichlet);
    st_stats.learned_used).  x_out (nullable) receives the solution. */
 int st_solve(st_ctx *c, int init, const double *x0, double *x_out, st_stats *stats);
 int st_get_solution(st_ctx *c, double *x);
+/* forget the held forward / adjoint solutions (a new run): ST_INIT_KEEP then starts
+   from the Dirichlet values and ST_INIT_LEARNED from the network's prediction alone.
+   No reference counterpart (the reference's GMRES always starts from zero). */
+int st_reset_solution(st_ctx *c);
+/* the design gradient dJ/drho (after the filter's chain rule) of the last st_to_step,
+   the reference's design_grad (FT/FluidTopoOpt3D.cpp:170-198) */
+int st_get_gradient(st_ctx *c, double *dc);
 /* replaces StokeFlowMapping3DFixed::applyMapping through LinearMapping
    (ST/StokeFlowMapping3D.cu:176-306, CX/LinearMapping.h:4-14): y = A x, host arrays */
 int st_apply(st_ctx *c, const double *x, double *y);
@@ -172,6 +179,11 @@ int st_profile_read(st_ctx *c, int kind, double *total_ms, long *launches, doubl
    channels) to the coarse state (p, u).  3D only. */
 /* record the pair (density used for the current alpha, current device solution) */
 int st_record_sample(st_ctx *c);
+/* the network's training loss on the current (design, solution) pair without
+   recording it, and the loss of the zero prediction on the same pair: how much of the
+   coarse state the trained network predicts on a design it has not seen.  No
+   reference counterpart. */
+int st_net_eval(st_ctx *c, double *loss, double *loss_zero);
 int st_num_samples(const st_ctx *c);
 /* Adam training on the recorded samples (batch 1); loss = mean squared error of the
    scaled coarse state after the last epoch */

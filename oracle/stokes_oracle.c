@@ -562,7 +562,7 @@ void orc_coarsen(int dim, int nx, int ny, int nz, const uint8_t *ffixed, const d
         for (int I = 0; I < A; I++) for (int J = 0; J < B; J++) for (int K = 0; K < C; K++) {
             long Q = c.nc + c.fo[d] + ((long)I * B + J) * C + K;
             int CI[3] = {I, J, K};
-            int allfix = 1; double sw = 0.0, sa = 0.0;
+            int allfix = 1, zero = 0; double sw = 0.0, sa = 0.0, sh = 0.0;
             /* tangential children: 2 per tangential axis (1 along z in 2D) */
             int t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2;
             for (int on = -1; on <= 1; on++) {
@@ -576,10 +576,14 @@ void orc_coarsen(int dim, int nx, int ny, int nz, const uint8_t *ffixed, const d
                     long q = f.nc + f.fo[d] + ((long)fi[0] * b_ + fi[1]) * c_ + fi[2];
                     if (on == 0) allfix &= ffixed[q];
                     sw += wn; sa += wn * fpen[q];
+                    if (fpen[q] > 0.0) sh += wn / fpen[q]; else zero = 1;
                 }
             }
             cfixed[Q] = (uint8_t)allfix;
-            cpen[Q] = allfix ? 0.0 : 4.0 * sa / sw;
+            /* midpoint of the weighted arithmetic and harmonic means (the two Wiener
+             * bounds of an effective coefficient), times (h_c/h_f)^2 = 4 */
+            const double ar = sa / sw, hm = (zero || sh <= 0.0) ? 0.0 : sw / sh;
+            cpen[Q] = allfix ? 0.0 : 4.0 * 0.5 * (ar + hm);
         }
     }
 }

@@ -67,6 +67,9 @@ SIGNATURES = {
     "st_set_rhs": (I, [C, D, D]),
     "st_solve": (I, [C, I, D, D, ctypes.POINTER(StStats)]),
     "st_get_solution": (I, [C, D]),
+    "st_reset_solution": (I, [C]),
+    "st_get_gradient": (I, [C, D]),
+    "st_net_eval": (I, [C, D, D]),
     "st_apply": (I, [C, D, D]),
     "st_apply_device": (I, [C, C, C]),
     "st_objective": (I, [C, D, D]),

@@ -14,7 +14,7 @@ struct AlphaParams {
 // one workgroup): pre-smoothing, residual and restriction down to the coarsest
 // level, its dense solve, then prolongation and post-smoothing back to l0
 constexpr int ST_MAX_LEVELS = 12;
-constexpr long ST_TAIL_MAX_CELLS = 4096;
+constexpr long ST_TAIL_MAX_CELLS = 512;  // 8^3: one cell per thread of the workgroup
 // (kept in device memory: the kernel indexes it by level; b and x of level 0 are
 // the caller's vectors when l0 = 0, the level buffers b[l], x[l] otherwise)
 template <typename TS> struct TailArgs {

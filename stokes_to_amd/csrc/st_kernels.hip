@@ -391,8 +391,8 @@ __global__ void __launch_bounds__(256) k_prolong(Lvl gf, Lvl gc, const uint8_t *
 }
 
 // coarse flags (free if any coincident child is free) and coarse face alpha
-// (4 x normal-linear weighted mean of the child faces; 4 = (h_c/h_f)^2 keeps
-// the penalty in unit-spacing form).  Reference coarsening:
+// (4 x the midpoint of the normal-linear weighted arithmetic and harmonic means of
+// the child faces; 4 = (h_c/h_f)^2 keeps the penalty in unit-spacing form).  Reference coarsening:
 // ST/StokeFlowSubSystem3D.cu:103-154.
 template <typename TF, typename TC, int DIM>
 __global__ void __launch_bounds__(256) k_coarsen(Lvl gf, Lvl gc, const uint8_t *flf, const TF *alf, uint8_t *flc,
@@ -411,8 +411,8 @@ __global__ void __launch_bounds__(256) k_coarsen(Lvl gf, Lvl gc, const uint8_t *
     int fdim[3] = {gf.nx, gf.ny, gf.nz};
 #pragma unroll
     for (int d = 0; d < DIM; d++) {
-        int fr = 0;
-        double sw = 0.0, sa = 0.0;
+        int fr = 0, zero = 0;
+        double sw = 0.0, sa = 0.0, sh = 0.0;
         for (int on = -1; on <= 1; on++) {
             const double w = (on == 0) ? 1.0 : 0.5;
             for (int a = 0; a < 2; a++)
@@ -429,12 +429,18 @@ __global__ void __launch_bounds__(256) k_coarsen(Lvl gf, Lvl gc, const uint8_t *
                     if (!ok) continue;
                     long q = f.id(fi[0], fi[1], fi[2]);
                     if (on == 0) fr |= (flf[q] >> (1 + d)) & 1;
+                    const double av = (double)Cvt<TF>::ld(alf[(long)d * gf.S + q]);
                     sw += w;
-                    sa += w * (double)Cvt<TF>::ld(alf[(long)d * gf.S + q]);
+                    sa += w * av;
+                    if (av > 0.0) sh += w / av;
+                    else zero = 1;
                 }
         }
         out |= fr << (1 + d);
-        alc[(long)d * gc.S + n] = Cvt<TC>::st((typename Cvt<TC>::C)((fr && sw > 0) ? 4.0 * sa / sw : 0.0));
+        // midpoint of the weighted arithmetic and harmonic child means (the two Wiener
+        // bounds of an effective coefficient), times (h_c/h_f)^2 = 4
+        const double ar = sw > 0 ? sa / sw : 0.0, hm = (zero || sh <= 0.0) ? 0.0 : sw / sh;
+        alc[(long)d * gc.S + n] = Cvt<TC>::st((typename Cvt<TC>::C)((fr && sw > 0) ? 2.0 * (ar + hm) : 0.0));
     }
     flc[n] = (uint8_t)out;
 }

@@ -36,6 +36,10 @@ int net_set_params(NetImpl *, const float *host);
 int net_add_sample(NetImpl *, const Lvl &gc, int nf, int ratio, const double *rho_dev, const uint8_t *flc,
                    const double *xdc, const double *xc);
 int net_num_samples(const NetImpl *);
+// loss of the network on one (design, state) pair that is not recorded, and the loss
+// of the zero prediction on the same pair (same scaling as training)
+int net_eval(NetImpl *, const Lvl &gc, int nf, int ratio, const double *rho_dev, const uint8_t *flc,
+             const double *xdc, const double *xc, double *loss, double *loss_zero);
 // train on the recorded samples; returns the mean loss of the last epoch
 int net_train(NetImpl *, int epochs, float lr, double *loss);
 // predicted coarse state (padded FP64 vector of level lc) for the given density

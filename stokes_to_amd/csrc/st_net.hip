@@ -675,7 +675,7 @@ NetImpl *net_create(int R, int max_samples, unsigned seed, hipStream_t s)
          n->alloc(&n->gP2, v2 * 32) && n->alloc(&n->gE2a, v1 * 32) && n->alloc(&n->gP1, v1 * 16) &&
          n->alloc(&n->gE1a, v0 * 16);
     ok = ok && n->alloc(&n->part, maxparts * n->nsplit) && n->alloc(&n->lpart, nblk(v0) + 1) &&
-         n->alloc(&n->feat, (long)(max_samples + 1) * v0 * 4) && n->alloc(&n->tgt_raw, (long)max_samples * v0 * 4 + 1) &&
+         n->alloc(&n->feat, (long)(max_samples + 1) * v0 * 4) && n->alloc(&n->tgt_raw, (long)(max_samples + 1) * v0 * 4) &&
          n->alloc(&n->tgt, v0 * 4) && n->alloc(&n->scale, NET_NOUT);
     if (!ok) {
         net_destroy(n);
@@ -764,6 +764,27 @@ int net_add_sample(NetImpl *n, const Lvl &gc, int nf, int ratio, const double *r
 }
 
 int net_num_samples(const NetImpl *n) { return n->nsamp < n->maxs ? n->nsamp : n->maxs; }
+
+int net_eval(NetImpl *n, const Lvl &gc, int nf, int ratio, const double *rho, const uint8_t *flc, const double *xdc,
+             const double *xc, double *loss, double *loss_zero)
+{
+    if (gc.nx != n->R || gc.dim != 3 || !n->scaled) return -1;
+    float *feat = n->feat + (long)n->maxs * n->nv0 * 4, *raw = n->tgt_raw + (long)n->maxs * n->nv0 * 4;
+    features(n, gc, nf, ratio, rho, flc, xdc, feat);
+    std::vector<float> one(NET_NOUT, 1.f);
+    hipMemcpyAsync(n->tgt, one.data(), sizeof(float) * NET_NOUT, hipMemcpyHostToDevice, n->s);
+    hipLaunchKernelGGL(k_targets, dim3(nblk(n->nv0)), dim3(256), 0, n->s, gc, 3, xc, n->tgt, raw);
+    hipLaunchKernelGGL(k_scale_targets, dim3(nblk(n->nv0 * 4)), dim3(256), 0, n->s, n->nv0 * 4, raw, n->scale, n->tgt);
+    n->forward();
+    *loss = n->loss_of(n->tgt);
+    std::vector<float> t((size_t)n->nv0 * 4);
+    hipMemcpyAsync(t.data(), n->tgt, sizeof(float) * t.size(), hipMemcpyDeviceToHost, n->s);
+    if (hipStreamSynchronize(n->s) != hipSuccess) return -1;
+    double z = 0.0;
+    for (float v : t) z += (double)v * v;
+    *loss_zero = 0.5 * z / (double)t.size();
+    return 0;
+}
 
 static void set_scales(NetImpl *n)
 {

@@ -73,7 +73,7 @@ struct st_ctx {
     const double *rho_used = nullptr;  // density the current alpha was computed from
     double *xdc = nullptr, *xcp = nullptr, *xcn = nullptr;  // level-lc Dirichlet state, two predictions
     double *rhop = nullptr;     // design of the solution held in X (for the learned correction)
-    bool have_rhop = false, have_lam = false;
+    bool have_rhop = false, have_lam = false, have_grad = false;
     // kernel timing: event pairs around finest-level launches
     bool prof = false;
     std::vector<hipEvent_t> evpool[ST_PROF_NKIND];
@@ -961,6 +961,22 @@ int st_solve(st_ctx *c, int init, const double *x0, double *x_out, st_stats *sta
     return rc;
 }
 
+int st_get_gradient(st_ctx *c, double *dc)
+{
+    if (!c || !dc) return ST_EINVAL;
+    if (!c->have_grad) return fail(c, ST_ESTATE, "no design gradient yet (st_to_step)");
+    CK(hipMemcpyAsync(dc, c->dc, sizeof(double) * c->ncell, hipMemcpyDeviceToHost, c->s));
+    CK(hipStreamSynchronize(c->s));
+    return ST_OK;
+}
+
+int st_reset_solution(st_ctx *c)
+{
+    if (!c) return ST_EINVAL;
+    c->have_x = c->have_lam = c->have_rhop = false;
+    return ST_OK;
+}
+
 int st_get_solution(st_ctx *c, double *x)
 {
     if (!c || !x) return ST_EINVAL;
@@ -1048,6 +1064,7 @@ static void filter_dev(st_ctx *c, const double *in, double *out, int transpose)
 int st_filter(st_ctx *c, const double *in, double *out, int transpose)
 {
     if (!c || !in || !out) return ST_EINVAL;
+    c->have_grad = false;  // c->dc is reused below
     CK(hipMemcpyAsync(c->rn, in, sizeof(double) * c->ncell, hipMemcpyHostToDevice, c->s));
     filter_dev(c, c->rn, c->dc, transpose);
     CK(hipMemcpyAsync(out, c->dc, sizeof(double) * c->ncell, hipMemcpyDeviceToHost, c->s));
@@ -1090,6 +1107,7 @@ int st_oc_update(st_ctx *c, const double *rho, const double *dcp, double volfrac
                  double rho_min, double *rho_new, double *change)
 {
     if (!c || !rho || !dcp || !rho_new) return ST_EINVAL;
+    c->have_grad = false;  // c->dc is reused below
     CK(hipMemcpyAsync(c->rn, rho, sizeof(double) * c->ncell, hipMemcpyHostToDevice, c->s));
     CK(hipMemcpyAsync(c->dc, dcp, sizeof(double) * c->ncell, hipMemcpyHostToDevice, c->s));
     CKR(oc_dev(c, c->rn, c->dc, volfrac, move, eta, rho_min, c->dj, change, nullptr));
@@ -1117,6 +1135,7 @@ int st_mma_update(st_ctx *c, double *x, const double *dfdx, double g, const doub
     CKR(check_finite(c, x, c->ncell, "x"));
     CKR(check_finite(c, dfdx, c->ncell, "dfdx"));
     CKR(mma_ready(c));
+    c->have_grad = false;  // c->dc is reused below
     const size_t nb = sizeof(double) * c->ncell;
     CK(hipMemcpyAsync(c->rn, x, nb, hipMemcpyHostToDevice, c->s));
     CK(hipMemcpyAsync(c->dc, dfdx, nb, hipMemcpyHostToDevice, c->s));
@@ -1207,6 +1226,7 @@ int st_to_step(st_ctx *c, int init, double volfrac, double move, double eta, dou
     c->have_lam = true;
     launch_sensitivity(c->lv[0], c->rhof, alpha_params(c), c->fl[0], c->X, c->LAM, c->dj, c->s);
     filter_dev(c, c->dj, c->dc, 1);
+    c->have_grad = true;
     if (c->prm.optimizer == ST_OPT_MMA)
         CKR(mma_dev(c, c->rho, c->dc, volfrac, move, rho_min, c->rn, change, vol));
     else
@@ -1226,6 +1246,19 @@ int st_record_sample(st_ctx *c)
     if (net_add_sample(c->net, c->lv[c->lc], c->n, c->n / c->lv[c->lc].nx, c->rho_used, c->fl[c->lc], c->xdc,
                        c->xcn) != 0)
         return fail(c, ST_EHIP, "recording a sample failed");
+    return ST_OK;
+}
+
+int st_net_eval(st_ctx *c, double *loss, double *loss_zero)
+{
+    if (!c || !loss || !loss_zero) return ST_EINVAL;
+    CKR(net_ready(c));
+    if (!c->have_x || !c->rho_used) return fail(c, ST_ESTATE, "evaluate after a solve of a design");
+    restrict_state(c, c->XD, c->xdc);
+    restrict_state(c, c->X, c->xcn);
+    if (net_eval(c->net, c->lv[c->lc], c->n, c->n / c->lv[c->lc].nx, c->rho_used, c->fl[c->lc], c->xdc, c->xcn, loss,
+                 loss_zero) != 0)
+        return fail(c, ST_ESTATE, "the predictor is not trained");
     return ST_OK;
 }
 

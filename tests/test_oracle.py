@@ -377,3 +377,60 @@ def test_mma_step_is_feasible_and_inside_the_box(orc):
         assert np.all(x >= lo - 1e-15) and np.all(x <= hi + 1e-15)
         assert np.abs(x - xo).max() <= 0.2 + 1e-12
     assert abs(x.mean() - 0.4) < 1e-6
+
+
+def test_cpu_configuration_2d_64_to_run(orc):
+    """The CPU-runnable configuration end to end on the oracle: 2D 64^2 channel,
+    initial density 0.15, volume fraction 0.15, filter radius 1.5, 10 OC iterations.
+    Invariants: the (filtered) volume is met at every step, designs stay in
+    [rho_min, 1], the change respects the move limit, and the dissipation drops."""
+    n, vf = 64, 0.15
+    P = channel_problem(2, n)
+    rho = np.full(n * n, vf)
+    Js = []
+    for it in range(10):
+        rn, J, ch, x, lam = orc.to_step(P, rho, volfrac=vf, rtol=1e-10)
+        rf = orc.filt(2, P.shape, 1.5, rn)
+        assert abs(rf.mean() - vf) < 1e-8
+        assert rn.min() >= 1e-3 - 1e-15 and rn.max() <= 1.0 + 1e-15
+        assert ch <= 0.2 + 1e-12 and abs(ch - np.abs(rn - rho).max()) < 1e-15
+        Js.append(J)
+        rho = rn
+    assert Js[-1] < 0.5 * Js[0]
+
+
+def test_coarse_alpha_is_the_midpoint_of_the_wiener_bounds(orc):
+    """Coarse face penalty = 4 x (weighted arithmetic + weighted harmonic mean)/2 of the
+    child faces: a uniform fine alpha a gives 4a; any child with alpha 0 (fluid) makes
+    the harmonic mean 0, so the coarse value is half the scaled arithmetic mean."""
+    dim, shape = 3, (8, 8, 8)
+    n, s = orc.sizes(dim, *shape)
+    nc = s[0]
+    fixed = np.zeros(n, np.uint8)
+    pen = np.zeros(n)
+    pen[nc:] = 0.7
+    cfix, cpen = orc.coarsen(dim, shape, fixed, pen)
+    ncc = orc.sizes(dim, 4, 4, 4)[1][0]
+    assert np.allclose(cpen[ncc:], 4 * 0.7, rtol=1e-14)
+    rng = np.random.default_rng(3)
+    pen[nc:] = rng.uniform(0.1, 1.0, n - nc)
+    pen[nc + 5] = 0.0                                   # one fluid child face
+    _, cpen2 = orc.coarsen(dim, shape, fixed, pen)
+    pen_nz = pen.copy()
+    pen_nz[nc + 5] = 1e-300
+    _, cpen3 = orc.coarsen(dim, shape, fixed, pen_nz)  # harmonic mean -> ~0 as well
+    assert np.allclose(cpen2, cpen3, rtol=1e-12, atol=1e-300)
+    # one coarse x-face by hand: children (2I+o, 2J+u, 2K+v), o in {-1,0,1} with weights
+    # 1/2, 1, 1/2, u, v in {0,1}
+    I, J, K = 1, 2, 1
+    fx = pen[nc:nc + 9 * 8 * 8].reshape(9, 8, 8)
+    w, a = [], []
+    for o, wo in ((-1, 0.5), (0, 1.0), (1, 0.5)):
+        for u in (0, 1):
+            for v in (0, 1):
+                w.append(wo)
+                a.append(fx[2 * I + o, 2 * J + u, 2 * K + v])
+    w, a = np.array(w), np.array(a)
+    expect = 4.0 * 0.5 * ((w * a).sum() / w.sum() + w.sum() / (w / a).sum())
+    got = cpen2[ncc:ncc + 5 * 4 * 4].reshape(5, 4, 4)[I, J, K]
+    assert abs(got - expect) < 1e-13 * expect
