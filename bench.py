@@ -41,6 +41,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# measured HBM traffic per launch of the 256^3 FP64 smoother colour pass: rocprofv3
+# FETCH_SIZE and WRITE_SIZE runs scaled by a 1 GiB read / write of known size
+# (profiles/k_march_smooth256_counters.md)
+SMOOTH256_FP64_TRAFFIC_BYTES = 2.94e9
 METRIC = "ms per Stokes solve to 1e-8 rel residual at 3D 256³: plain FP64 MG vs learned+MP"
 
 
@@ -99,14 +103,15 @@ def summary(stats):
     }
 
 
-def roofline_of(prof):
+def roofline_of(prof, n):
     pr = prof["smooth"]
     avg_ms = pr["total_ms"] / max(pr["launches"], 1)
     achieved = pr["bytes"] / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
     return {
         "bound": "hbm", "kernel": "k_march<double,SMOOTH> (256^3 finest-level colour pass, plain FP64 path)",
         "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": SMOOTH256_FP64_TRAFFIC_BYTES if n == 256 else None,
         "avg_launch_ms": round(avg_ms, 4), "launches": pr["launches"],
         "algorithmic_bytes_per_launch": pr["bytes"],
     }
@@ -262,7 +267,7 @@ def main():
             "learned": {k: extra[k] for k in ("train_ms_total", "train_ms_per_iter", "inference_ms_median",
                                               "learned_taken", "loss_vs_zero_median")},
             "final_objective": extra["final_J"], "final_volume": extra["final_vol"],
-            "roofline": roofline_of(extra["prof"]), "kernels_plain_fp64": extra["prof"],
+            "roofline": roofline_of(extra["prof"], args.n), "kernels_plain_fp64": extra["prof"],
         })
         base_design, base_iters = extra["last_design"], int(np.median(plain["iters"]))
         log(args, f"[bench] plain {plain['median_ms']:.1f} ms, keep+{args.mp} {keep['median_ms']:.1f} ms, "
@@ -278,7 +283,7 @@ def main():
                                    f"{args.mean}, seed 12345), rtol {args.rtol} true FP64 residual",
                        "n": args.n, "dofs": int(prob.fixed.size), "rtol": args.rtol, "mp": args.mp},
             "plain_fp64": p, "mixed": m, "speedup_vs_plain": round(p["median_ms"] / value, 3),
-            "roofline": roofline_of(pp), "kernels": {"fp64": pp, args.mp: mp},
+            "roofline": roofline_of(pp, args.n), "kernels": {"fp64": pp, args.mp: mp},
         })
         base_design, base_iters = rho, int(np.median(p["iters"]))
         log(args, f"[bench] fp64 {p['median_ms']:.1f} ms, {args.mp} {m['median_ms']:.1f} ms")

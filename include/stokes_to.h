@@ -110,7 +110,8 @@ int st_set_rhs(st_ctx *c, const double *body, const double *dirichlet);
    corrected by the predicted change of the coarse state between its design and this
    one; without one, the Dirichlet values plus the predicted coarse state; taken only
    if its true FP64 residual is below that of the candidate without the network, see
-   st_stats.learned_used).  x_out (nullable) receives the solution. */
+   st_stats.learned_used; after 3 rejections in a row the network's candidate is only
+   built on every 4th call).  x_out (nullable) receives the solution. */
 int st_solve(st_ctx *c, int init, const double *x0, double *x_out, st_stats *stats);
 int st_get_solution(st_ctx *c, double *x);
 /* forget the held forward / adjoint solutions (a new run): ST_INIT_KEEP then starts

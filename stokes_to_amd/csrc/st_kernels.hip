@@ -1232,6 +1232,7 @@ INST_PAIR(float, bf16)
     template void launch_multiaxpy<W, V>(long, W *, const V *, long, int, const double *, double *, hipStream_t);
 INST_KRYLOV(double, double)
 INST_KRYLOV(double, float)
+INST_KRYLOV(float, float)
 INST_PAIR(bf16, float)
 
 }  // namespace st

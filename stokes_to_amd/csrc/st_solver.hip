@@ -74,6 +74,8 @@ struct st_ctx {
     double *xdc = nullptr, *xcp = nullptr, *xcn = nullptr;  // level-lc Dirichlet state, two predictions
     double *rhop = nullptr;     // design of the solution held in X (for the learned correction)
     bool have_rhop = false, have_lam = false, have_grad = false;
+    int learned_rejects = 0;  // consecutive rejections of the network's candidate
+    long learned_calls = 0;
     // kernel timing: event pairs around finest-level launches
     bool prof = false;
     std::vector<hipEvent_t> evpool[ST_PROF_NKIND];
@@ -355,6 +357,12 @@ static void precondition(st_ctx *c, const float *v, float *z)
     }
 }
 
+// w = A z inside FGMRES: the FP32 operator for the FP32 path
+static void inner_apply(st_ctx *c, const float *z, float *w)
+{
+    launch_apply<float>(c->lv[0], c->fl[0], c->h32.al[0], z, w, c->s);
+}
+
 // w = A z with the FP64 operator
 static void apply64(st_ctx *c, const double *z, double *w)
 {
@@ -371,16 +379,27 @@ static void apply64(st_ctx *c, const float *z, double *w)
     }
 }
 
+static void inner_apply(st_ctx *c, const double *z, double *w) { apply64(c, z, w); }
+static void inner_apply(st_ctx *c, const float *z, double *w) { apply64(c, z, w); }
+
 // ----------------------------------------------------------------- FGMRES
 
 // solves A x = b (padded FP64, device), x holds the initial guess on entry.
-// TV: storage of the Krylov vectors V and Z (FP64 or FP32); x, w, the inner products
-// and the stopping residual are FP64 (the true residual is recomputed at every
-// restart, which bounds the effect of FP32 storage to one restart cycle)
-template <typename TV> static int fgmres(st_ctx *c, const double *b, double *x, st_stats *st)
+// TV: storage of the Krylov vectors V and Z; TW: precision of the inner operator
+// w = A z and of w.  (double, double): the plain FP64 method.  (float, float): the
+// FP32 path, an FP32 inner FGMRES -- operator, basis and preconditioner in FP32, inner
+// products accumulated in FP64 -- inside FP64 iterative refinement: x, its update and
+// the residual that starts every restart cycle and decides convergence are FP64.
+// (float, double): the BF16 path (FP64 operator on FP32 Krylov vectors).  With an
+// FP32 operator the Arnoldi estimate of the residual is only good to ~1e-6 of the
+// cycle's starting residual, so a cycle also ends there and the FP64 residual of the
+// restart takes over.
+template <typename TV, typename TW> static int fgmres(st_ctx *c, const double *b, double *x, st_stats *st)
 {
     TV *const Vb = reinterpret_cast<TV *>(c->V);
     TV *const Zs = reinterpret_cast<TV *>(c->Zb);
+    TW *const Wk = reinterpret_cast<TW *>(c->Wv);
+    const double floor_rel = std::is_same<TW, float>::value ? 1e-6 : 0.0;
     const Lvl &g = c->lv[0];
     const long nv = c->nvec;
     const int m = c->prm.restart;
@@ -395,7 +414,7 @@ template <typename TV> static int fgmres(st_ctx *c, const double *b, double *x, 
     double rr = 0.0, rr0 = -1.0;
     const int rb = (int)residual_blocks(g);
     for (;;) {
-        double *r0 = std::is_same<TV, double>::value ? c->V : c->Wv;
+        double *r0 = std::is_same<TV, double>::value ? c->V : c->Wv;  // FP64 residual
         launch_residual<double>(g, c->fl[0], c->h64.al[0], b, x, r0, c->part, c->s);
         launch_sum(rb, 1, c->part, c->red, c->s);
         double r2 = 0.0;
@@ -409,17 +428,18 @@ template <typename TV> static int fgmres(st_ctx *c, const double *b, double *x, 
         launch_axpby<double, TV>(nv, 1.0 / beta, r0, 0.0, Vb, c->s);
         std::fill(gv.begin(), gv.end(), 0.0);
         gv[0] = beta;
+        const double cycle_tol = std::max(rtol * bn, floor_rel * beta);
         int j = 0;
         for (; j < m && it < c->prm.max_iter; j++) {
             TV *vj = Vb + (long)j * nv, *zj = Zs + (long)j * nv;
             precondition(c, vj, zj);
             prof_mark(c, ST_PROF_APPLY);
-            apply64(c, zj, c->Wv);
+            inner_apply(c, zj, Wk);
             prof_mark(c, ST_PROF_APPLY);
             const unsigned nbk = blas_blocks(nv);
-            launch_multidot<double, TV>(nv, c->Wv, Vb, nv, j + 1, c->part, c->s);
+            launch_multidot<TW, TV>(nv, Wk, Vb, nv, j + 1, c->part, c->s);
             launch_sum((int)nbk, j + 1, c->part, c->h_dev, c->s);
-            launch_multiaxpy<double, TV>(nv, c->Wv, Vb, nv, j + 1, c->h_dev, c->part, c->s);
+            launch_multiaxpy<TW, TV>(nv, Wk, Vb, nv, j + 1, c->h_dev, c->part, c->s);
             launch_sum((int)nbk, 1, c->part, c->h_dev + (j + 1), c->s);
             CK(hipMemcpyAsync(hh.data(), c->h_dev, sizeof(double) * (j + 2), hipMemcpyDeviceToHost, c->s));
             CK(hipStreamSynchronize(c->s));
@@ -427,7 +447,7 @@ template <typename TV> static int fgmres(st_ctx *c, const double *b, double *x, 
             if (!std::isfinite(hn)) return fail(c, ST_ENAN, "non-finite Arnoldi vector");
             for (int i = 0; i <= j; i++) H[(size_t)i * m + j] = hh[i];
             H[(size_t)(j + 1) * m + j] = hn;
-            if (hn > 0) launch_axpby<double, TV>(nv, 1.0 / hn, c->Wv, 0.0, Vb + (long)(j + 1) * nv, c->s);
+            if (hn > 0) launch_axpby<TW, TV>(nv, 1.0 / hn, Wk, 0.0, Vb + (long)(j + 1) * nv, c->s);
             for (int i = 0; i < j; i++) {
                 double a = H[(size_t)i * m + j], bb = H[(size_t)(i + 1) * m + j];
                 H[(size_t)i * m + j] = cs[i] * a + sn[i] * bb;
@@ -442,7 +462,7 @@ template <typename TV> static int fgmres(st_ctx *c, const double *b, double *x, 
             gv[j + 1] = -sn[j] * gv[j];
             gv[j] = cs[j] * gv[j];
             it++;
-            if (std::fabs(gv[j + 1]) / bn <= rtol || hn == 0.0) {
+            if (std::fabs(gv[j + 1]) <= cycle_tol || hn == 0.0) {
                 j++;
                 break;
             }
@@ -785,7 +805,9 @@ static int solve_core(st_ctx *c, const double *b, double *x, const double *xd, i
     if (c->dirty) CKR(setup_hierarchy(c));
     local.setup_ms = c->last_setup_ms;
     if (init == ST_INIT_ZERO) launch_axpby<double, double>(c->nvec, 1.0, xd, 0.0, x, c->s);
-    int rc = (c->prm.precision == ST_FP64) ? fgmres<double>(c, b, x, &local) : fgmres<float>(c, b, x, &local);
+    int rc = (c->prm.precision == ST_FP64)   ? fgmres<double, double>(c, b, x, &local)
+             : (c->prm.precision == ST_FP32) ? fgmres<float, float>(c, b, x, &local)
+                                              : fgmres<float, double>(c, b, x, &local);
     CK(hipEventRecord(c->e1, c->s));
     CK(hipEventSynchronize(c->e1));
     float ms = 0.f;
@@ -872,7 +894,9 @@ static void remember_design(st_ctx *c)
 //   without one:
 //     x_D + P P net(rho)                   against  x_D (the zero initial guess).
 // The candidate with the smaller true FP64 residual ||b - A x|| is kept, so the
-// plain method remains the judge of every prediction.
+// plain method remains the judge of every prediction.  After 3 rejections in a row
+// the candidate is built and checked only on every 4th call (the others take the
+// start without the network), so a predictor that keeps losing costs little.
 static int learned_guess(st_ctx *c, float *ms, int *used)
 {
     CKR(net_ready(c));
@@ -887,8 +911,10 @@ static int learned_guess(st_ctx *c, float *ms, int *used)
     const bool corr = c->have_x && c->have_rhop;
     if (!c->have_x) launch_axpby<double, double>(c->nvec, 1.0, c->XD, 0.0, c->X, c->s);
     int take = 0;
-    if (c->have_x && !c->have_rhop) {
-        take = 0;  // no design for the held solution: plain warm start
+    c->learned_calls++;
+    const bool skip = c->have_x && c->learned_rejects >= 3 && (c->learned_calls % 4) != 0;
+    if ((c->have_x && !c->have_rhop) || skip) {
+        take = 0;  // no design for the held solution, or backing off: plain warm start
     } else {
         CKR(predict(c, c->rho_used, c->xcn));
         if (corr) {
@@ -902,6 +928,7 @@ static int learned_guess(st_ctx *c, float *ms, int *used)
         CKR(resid_norm(c, c->X, &r_cand));
         take = (std::isfinite(r_cand) && r_cand < r_base) ? 1 : 0;
         if (!take) launch_axpby<double, double>(c->nvec, 1.0, c->Wv, 0.0, c->X, c->s);
+        c->learned_rejects = take ? 0 : c->learned_rejects + 1;
     }
     CK(hipEventRecord(b, c->s));
     CK(hipEventSynchronize(b));
