@@ -53,7 +53,8 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10, help="timed design iterations (solves)")
     ap.add_argument("--warmup", type=int, default=15, help="untimed iterations (online training window)")
-    ap.add_argument("--workload", default="to", choices=["to", "grf"])
+    ap.add_argument("--workload", default="to", choices=["to", "grf", "to-pair"])
+    ap.add_argument("--iters", type=int, default=80, help="design iterations of the to-pair workload")
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--volfrac", type=float, default=0.2)
     ap.add_argument("--mean", type=float, default=0.2, help="GRF mean (grf workload)")
@@ -172,6 +173,36 @@ def run_to(args, prob):
     return summary(plain), summary(learned), summary(keep), extra
 
 
+def run_to_pair(args, prob):
+    """Two independent TO runs of --iters iterations: plain FP64 (zero starts) and
+    learned + MP (online training on the first --warmup iterations); per-iteration
+    forward-solve times and the L2 difference of the final designs."""
+    import torch
+    P = new_ctx(args, prob, "fp64")
+    L = new_ctx(args, prob, args.mp)
+    rho0 = np.full(args.n ** 3, args.volfrac)
+    P.set_design(rho0)
+    L.set_design(rho0)
+    tp, tl = [], []
+    for it in range(args.iters):
+        rp = P.to_step(args.volfrac, init="zero")
+        rl = L.to_step(args.volfrac, init="learned")
+        tp.append(rp["fwd"]["ms"])
+        tl.append(rl["fwd"]["ms"])
+        if it < args.warmup:
+            L.record_sample()
+            L.net_train(args.epochs, args.lr)
+        log(args, f"[bench] it {it:2d} plain {rp['fwd']['iters']} it {rp['fwd']['ms']:.1f} ms J {rp['J']:.8g} | "
+                  f"learned+{args.mp} {rl['fwd']['iters']} it {rl['fwd']['ms']:.1f} ms J {rl['J']:.8g}")
+    torch.cuda.synchronize()
+    dp, dl = P.get_design(), L.get_design()
+    diff = float(np.linalg.norm(dp - dl) / np.linalg.norm(dp))
+    P.close()
+    L.close()
+    return {"plain_ms": tp, "learned_ms": tl, "final_design_rel_l2": diff,
+            "final_J_plain": rp["J"], "final_J_learned": rl["J"]}
+
+
 def run_grf(args, prob):
     import torch
     from stokes_to_amd.problems import grf_density
@@ -249,6 +280,23 @@ def main():
 
     out = {"metric": METRIC, "unit": "ms/solve", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
            "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "data": "synthetic"}
+    if args.workload == "to-pair":
+        r = run_to_pair(args, prob)
+        tail = slice(args.warmup, None)
+        value = float(np.median(r["learned_ms"][tail]))
+        out.update({
+            "value": round(value, 3), "ms_per_step": round(value, 3), "dtype": f"f64+{args.mp}",
+            "config": {"workload": f"two independent 3D {args.n}^3 TO runs of {args.iters} iterations (plain FP64 "
+                                   f"zero start vs learned+{args.mp}), volume fraction {args.volfrac}",
+                       "n": args.n, "iters": args.iters},
+            "plain_median_ms": float(np.median(r["plain_ms"][tail])),
+            "speedup_vs_plain": round(float(np.median(r["plain_ms"][tail])) / value, 3),
+            "final_design_rel_l2": r["final_design_rel_l2"],
+            "final_J_plain": r["final_J_plain"], "final_J_learned": r["final_J_learned"],
+            "plain_ms": r["plain_ms"], "learned_ms": r["learned_ms"], "roofline": None, "cpu_baseline": None,
+        })
+        print(json.dumps(out), flush=True)
+        return 0
     if args.workload == "to":
         plain, learned, keep, extra = run_to(args, prob)
         value = learned["median_ms"]

@@ -17,9 +17,16 @@ constexpr int ST_MAX_LEVELS = 12;
 constexpr long ST_TAIL_MAX_CELLS = 512;  // 8^3: one cell per thread of the workgroup
 // (kept in device memory: the kernel indexes it by level; b and x of level 0 are
 // the caller's vectors when l0 = 0, the level buffers b[l], x[l] otherwise)
+constexpr int ST_TAIL_LDS_BYTES = 150 * 1024;  // LDS pool of the LDS-resident variant
+struct TailLds {  // byte offsets of the level buffers in the LDS pool
+    int fl[ST_MAX_LEVELS], al[ST_MAX_LEVELS], b[ST_MAX_LEVELS], x[ST_MAX_LEVELS], t[ST_MAX_LEVELS],
+        r[ST_MAX_LEVELS];
+    int bs, bytes, ok;
+};
 template <typename TS> struct TailArgs {
     int l0, L, nu_pre, nu_post, nd, dim;
     float omega;
+    TailLds lds;
     Lvl g[ST_MAX_LEVELS];
     const uint8_t *fl[ST_MAX_LEVELS];
     const TS *al[ST_MAX_LEVELS];
@@ -28,7 +35,7 @@ template <typename TS> struct TailArgs {
     const double *minv;
 };
 template <typename TS>
-void launch_vcycle_tail(const TailArgs<TS> *dev_args, int dim, const TS *b0, TS *x0, int nd, hipStream_t);
+void launch_vcycle_tail(const TailArgs<TS> *dev_args, int dim, const TS *b0, TS *x0, int nd, bool lds, hipStream_t);
 
 template <typename TS> void launch_apply(const Lvl &, const uint8_t *, const TS *, const TS *, TS *, hipStream_t);
 template <typename TS>

@@ -631,6 +631,99 @@ __global__ void __launch_bounds__(512) k_vcycle_tail(const TailArgs<TS> *__restr
     }
 }
 
+// The same cycle with every level buffer staged in LDS (the levels of the tail
+// fit: ~136 KB in FP64 from 8^3 down): flags, alpha and the right-hand side are
+// loaded once, all passes read and write LDS, x of level l0 is written back.
+template <typename TS, int DIM>
+__global__ void __launch_bounds__(512) k_vcycle_tail_lds(const TailArgs<TS> *__restrict__ a, const TS *b0g, TS *x0g)
+{
+    __shared__ __align__(16) unsigned char pool[ST_TAIL_LDS_BYTES];
+    __shared__ const uint8_t *Pfl[ST_MAX_LEVELS];
+    __shared__ const TS *Pal[ST_MAX_LEVELS];
+    __shared__ TS *Pb[ST_MAX_LEVELS], *Px[ST_MAX_LEVELS], *Pt[ST_MAX_LEVELS], *Pr[ST_MAX_LEVELS];
+    const long T = blockDim.x;
+    const int l0 = a->l0, L = a->L;
+    const float om = a->omega;
+    if (threadIdx.x == 0) {
+        for (int l = l0; l < L; l++) {
+            Pfl[l] = (const uint8_t *)(pool + a->lds.fl[l]);
+            Pal[l] = (const TS *)(pool + a->lds.al[l]);
+            Pb[l] = (TS *)(pool + a->lds.b[l]);
+            Px[l] = (TS *)(pool + a->lds.x[l]);
+            Pt[l] = (TS *)(pool + a->lds.t[l]);
+            Pr[l] = (TS *)(pool + a->lds.r[l]);
+        }
+    }
+    for (long q = threadIdx.x; q < a->lds.bytes / 4; q += T) ((unsigned *)pool)[q] = 0u;  // padding = 0
+    __syncthreads();
+    for (int l = l0; l < L; l++) {
+        const Lvl g = a->g[l];
+        uint8_t *fl = (uint8_t *)Pfl[l];
+        TS *al = (TS *)Pal[l];
+        for (long q = threadIdx.x; q < g.S; q += T) fl[q] = a->fl[l][q];
+        for (long q = threadIdx.x; q < (long)g.dim * g.S; q += T) al[q] = a->al[l][q];
+    }
+    {
+        const Lvl g = a->g[l0];
+        const TS *bsrc = (l0 == 0) ? b0g : a->b[l0];
+        for (long q = threadIdx.x; q < (long)g.nfield * g.S; q += T) Pb[l0][q] = bsrc[q];
+    }
+    __syncthreads();
+    auto smooth = [&](int l, int nu, int first, bool zero) {
+        const Lvl g = a->g[l];
+        const uint8_t *fl = Pfl[l];
+        const TS *al = Pal[l], *b = Pb[l];
+        TS *x = Px[l], *t = Pt[l];
+        const long nc = (long)g.nx * g.ny * g.nz;
+        for (int sw = 0; sw < nu; sw++) {
+            if (zero && sw == 0)
+                for (long q = threadIdx.x; q < nc; q += T) smooth_body<TS, DIM, true>(g, fl, al, b, x, t, first, om, q);
+            else
+                for (long q = threadIdx.x; q < nc; q += T) smooth_body<TS, DIM, false>(g, fl, al, b, x, t, first, om, q);
+            __syncthreads();
+            for (long q = threadIdx.x; q < nc; q += T) smooth_body<TS, DIM, false>(g, fl, al, b, t, x, first ^ 1, om, q);
+            __syncthreads();
+        }
+    };
+    for (int l = l0; l + 1 < L; l++) {
+        const Lvl g = a->g[l];
+        if (a->nu_pre > 0) {
+            smooth(l, a->nu_pre, 0, true);
+        } else {
+            for (long q = threadIdx.x; q < (long)g.nfield * g.S; q += T) Px[l][q] = Cvt<TS>::st((typename Cvt<TS>::C)0);
+            __syncthreads();
+        }
+        for (long n = threadIdx.x; n < g.S; n += T) residual_body<TS, DIM>(g, Pfl[l], Pal[l], Pb[l], Px[l], Pr[l], n);
+        __syncthreads();
+        const Lvl gc = a->g[l + 1];
+        for (long n = threadIdx.x; n < gc.S; n += T) restrict_body<TS, DIM>(g, gc, Pfl[l + 1], Pr[l], Pb[l + 1], n);
+        __syncthreads();
+    }
+    const int Lc = L - 1, nd = a->nd;
+    double *bs = (double *)(pool + a->lds.bs);
+    const long *dofs = a->dofs;
+    for (int c = threadIdx.x; c < nd; c += T) bs[c] = (double)Cvt<TS>::ld(Pb[Lc][dofs[c]]);
+    __syncthreads();
+    for (int r = threadIdx.x; r < nd; r += T) {
+        double sacc = 0.0;
+        const double *row = a->minv + (long)r * nd;
+        for (int c = 0; c < nd; c++) sacc += row[c] * bs[c];
+        Px[Lc][dofs[r]] = Cvt<TS>::st((typename Cvt<TS>::C)sacc);
+    }
+    __syncthreads();
+    for (int l = L - 2; l >= l0; l--) {
+        const Lvl g = a->g[l];
+        for (long n = threadIdx.x; n < g.S; n += T) prolong_body<TS, DIM>(g, a->g[l + 1], Pfl[l], Px[l + 1], Px[l], n);
+        __syncthreads();
+        if (a->nu_post > 0) smooth(l, a->nu_post, 1, false);
+    }
+    {
+        const Lvl g = a->g[l0];
+        TS *xdst = (l0 == 0) ? x0g : a->x[l0];
+        for (long q = threadIdx.x; q < (long)g.nfield * g.S; q += T) xdst[q] = Px[l0][q];
+    }
+}
+
 // --------------------------------------------------- ABI <-> padded layout
 
 // field f of the ABI dof vector: dims (nx+(f==1), ny+(f==2), nz+(f==3)),
@@ -1184,8 +1277,13 @@ void launch_fill(long n, TS *y, double v, hipStream_t s)
 }
 
 template <typename TS>
-void launch_vcycle_tail(const TailArgs<TS> *a, int dim, const TS *b0, TS *x0, int nd, hipStream_t s)
+void launch_vcycle_tail(const TailArgs<TS> *a, int dim, const TS *b0, TS *x0, int nd, bool lds, hipStream_t s)
 {
+    if (lds) {
+        if (dim == 3) hipLaunchKernelGGL((k_vcycle_tail_lds<TS, 3>), dim3(1), dim3(512), 0, s, a, b0, x0);
+        else hipLaunchKernelGGL((k_vcycle_tail_lds<TS, 2>), dim3(1), dim3(512), 0, s, a, b0, x0);
+        return;
+    }
     const size_t sh = sizeof(double) * (size_t)(nd > 0 ? nd : 1);
     if (dim == 3) hipLaunchKernelGGL((k_vcycle_tail<TS, 3>), dim3(1), dim3(512), sh, s, a, b0, x0);
     else hipLaunchKernelGGL((k_vcycle_tail<TS, 2>), dim3(1), dim3(512), sh, s, a, b0, x0);
@@ -1207,7 +1305,7 @@ void launch_vcycle_tail(const TailArgs<TS> *a, int dim, const TS *b0, TS *x0, in
     template void launch_unpack<T>(const Lvl &, const T *, double *, hipStream_t);                          \
     template void launch_alpha_from_pen<T>(const Lvl &, const double *, const uint8_t *, T *, hipStream_t);  \
     template void launch_fill<T>(long, T *, double, hipStream_t);                                           \
-    template void launch_vcycle_tail<T>(const TailArgs<T> *, int, const T *, T *, int, hipStream_t);
+    template void launch_vcycle_tail<T>(const TailArgs<T> *, int, const T *, T *, int, bool, hipStream_t);
 
 INST_TS(double)
 INST_TS(float)

@@ -29,6 +29,7 @@ namespace {
 template <typename TS> struct Hier {
     std::vector<TS *> al, b, x, t, r;
     TailArgs<TS> *tail = nullptr;  // device copy of the fused coarse V-cycle's arguments
+    bool tail_lds = false;         // its levels fit the LDS pool
 };
 
 struct Net;  // learned predictor (st_net.hip)
@@ -303,6 +304,29 @@ template <typename TS> static void vcycle(st_ctx *c, const TS *b0, TS *x0)
             }
             a.dofs = c->d_dofs;
             a.minv = c->d_minv;
+            // LDS layout of the staged variant (16-byte aligned buffers)
+            {
+                long off = 0;
+                auto take = [&](long bytes) {
+                    const long o = off;
+                    off += (bytes + 15) / 16 * 16;
+                    return (int)o;
+                };
+                for (int l = lt; l < L; l++) {
+                    const Lvl &g = c->lv[l];
+                    const long nv = (long)g.nfield * g.S;
+                    a.lds.fl[l] = take(g.S);
+                    a.lds.al[l] = take((long)g.dim * g.S * sizeof(TS));
+                    a.lds.b[l] = take(nv * sizeof(TS));
+                    a.lds.x[l] = take(nv * sizeof(TS));
+                    a.lds.t[l] = take(nv * sizeof(TS));
+                    a.lds.r[l] = take(nv * sizeof(TS));
+                }
+                a.lds.bs = take((long)c->nd * sizeof(double));
+                a.lds.bytes = (int)off;
+                a.lds.ok = off <= ST_TAIL_LDS_BYTES;
+            }
+            h.tail_lds = a.lds.ok != 0;
             void *p = nullptr;
             if (hipMalloc(&p, sizeof(a)) == hipSuccess && hipMemcpy(p, &a, sizeof(a), hipMemcpyHostToDevice) == hipSuccess) {
                 c->allocs.push_back(p);
@@ -310,7 +334,8 @@ template <typename TS> static void vcycle(st_ctx *c, const TS *b0, TS *x0)
             }
         }
         if (h.tail) {
-            launch_vcycle_tail<TS>(h.tail, c->dim, lt == 0 ? b0 : h.b[lt], lt == 0 ? x0 : h.x[lt], c->nd, c->s);
+            launch_vcycle_tail<TS>(h.tail, c->dim, lt == 0 ? b0 : h.b[lt], lt == 0 ? x0 : h.x[lt], c->nd, h.tail_lds,
+                                   c->s);
         } else {  // allocation failed: per-level kernels (same arithmetic)
             for (int l = lt; l + 1 < L; l++) {
                 const Lvl &g = c->lv[l];

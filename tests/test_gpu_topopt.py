@@ -78,3 +78,24 @@ def test_to_loop_mma_matches_oracle(make_solver, orc, radius):
         assert abs(out["J"] - J_o) <= 1e-6 * abs(J_o), (it, out["J"], J_o)
         assert abs(out["change"] - ch_o) <= 1e-6 + 1e-4 * ch_o
     assert rel(s.get_design(), rho_o) < 1e-4
+
+
+def test_learned_mp_run_reaches_the_plain_design(make_solver):
+    """Two independent TO runs on the same problem: the plain FP64 method (zero
+    starts) and learned + FP32 multigrid (online training on the first iterations,
+    learned starts).  Both solve to 1e-8 true FP64 residual, so they must reach the
+    same design: final designs within 1e-4 relative L2 and objectives within 1e-6."""
+    dim, n, iters, vf = 3, 32, 12, 0.25
+    P = channel_problem(dim, n)
+    rho0 = np.full(n ** dim, vf)
+    a = solver_for(make_solver, P, rho0, rtol=1e-8)
+    b = solver_for(make_solver, P, rho0, rtol=1e-8, precision="fp32")
+    for it in range(iters):
+        ra = a.to_step(vf, init="zero")
+        rb = b.to_step(vf, init="learned")
+        assert ra["fwd"]["converged"] and rb["fwd"]["converged"]
+        if it < 5:
+            b.record_sample()
+            b.net_train(20, 1e-3)
+    assert abs(ra["J"] - rb["J"]) <= 1e-6 * abs(ra["J"])
+    assert rel(b.get_design(), a.get_design()) < 1e-4
