@@ -223,6 +223,7 @@ def run_grf(args, prob):
                 s.lib.st_profile(s.h, 1)
             s._check(s.lib.st_set_design_device(s.h, ptr))
             _, st = s.solve(init="zero", fetch=False)
+            log(args, f"[bench] grf {prec} solve {i}: {st['iters']} it, {st['ms']:.1f} ms, relres {st['relres']:.2e}")
             if i >= 1:
                 stats.append(st)
         torch.cuda.synchronize()

@@ -19,8 +19,13 @@
 namespace st {
 
 namespace {
-constexpr int BJ = 4, BK = 64, LJ = BJ + 3, LK = BK + 3, NL = LJ * LK;
+// tile shape (j rows x k columns, BJ * BK = 256 threads); BK is a power of two
+#ifndef ST_MARCH_BJ
+#define ST_MARCH_BJ 4
+#endif
+constexpr int BJ = ST_MARCH_BJ, BK = 256 / ST_MARCH_BJ, LJ = BJ + 3, LK = BK + 3, NL = LJ * LK;
 constexpr int NT = BJ * BK;
+static_assert(NL <= 2 * NT, "a plane's staged nodes are fetched two per thread");
 enum { M_APPLY = 0, M_RESID = 1, M_SMOOTH = 2 };
 
 __device__ __forceinline__ int slot3(int s) { return (s + 3) % 3; }
@@ -244,6 +249,16 @@ __global__ void __launch_bounds__(256) k_march(Lvl g, const uint8_t *__restrict_
             if (lastk) emit_edge(cown + 1, q + 1);
             if (lastj) emit_edge(cown + LK, q + sJ);
             if (lastk && lastj) emit_edge(cown + LK + 1, q + sJ + 1);
+            // the row padding k in (nz, Pz) is written too (zeros), so that y covers
+            // every entry of the vector: callers reuse buffers across precisions
+            if (k0 + BK == g.nz && g.nz + 1 + tk < g.Pz) {
+                const long qp = (long)i * sI + (long)j * sJ + (g.nz + 1 + tk);
+#pragma unroll
+                for (int f = 0; f < 4; f++) {
+                    y[(long)f * S + qp] = Cvt<TS>::st(C(0));
+                    if (lastj) y[(long)f * S + qp + sJ] = Cvt<TS>::st(C(0));
+                }
+            }
         } else {
             // ---- one colour of the box-Vanka smoother for the cells of plane i
             const int c = cown;
@@ -335,7 +350,7 @@ __global__ void __launch_bounds__(256) k_march(Lvl g, const uint8_t *__restrict_
     }
 }
 
-bool march_ok(const Lvl &g) { return g.dim == 3 && g.nz % BK == 0 && g.ny % BJ == 0 && g.nx >= 8; }
+bool march_ok(const Lvl &g) { return g.dim == 3 && g.nz >= 64 && g.nz % BK == 0 && g.ny % BJ == 0 && g.nx >= 8; }
 
 static dim3 march_grid(const Lvl &g, int &chunk)
 {

@@ -139,7 +139,14 @@ template <typename T> static int dalloc(st_ctx *c, T **p, long n)
         c->err = std::string("hipMalloc(") + std::to_string((long long)n * sizeof(T)) + " B): " + hipGetErrorString(e);
         return ST_EHIP;
     }
-    hipMemsetAsync(q, 0, (size_t)(n > 0 ? n : 1) * sizeof(T), c->s);
+    // padding entries (outside a field's valid range) start at 0 and no kernel writes
+    // anything else there
+    e = hipMemsetAsync(q, 0, (size_t)(n > 0 ? n : 1) * sizeof(T), c->s);
+    if (e != hipSuccess) {
+        hipFree(q);
+        c->err = std::string("hipMemsetAsync: ") + hipGetErrorString(e);
+        return ST_EHIP;
+    }
     c->allocs.push_back(q);
     *p = (T *)q;
     return ST_OK;
