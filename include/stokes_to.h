@@ -210,6 +210,9 @@ int st_mg_vcycle(st_ctx *c, const double *b, double *z);
 int st_mg_level_desc(st_ctx *c, int level, uint8_t *fixed, double *alpha_faces);
 /* one smoother colour step on `level`: xout = S_color(b, xin) */
 int st_mg_smooth(st_ctx *c, int level, int color, const double *b, const double *xin, double *xout);
+/* the same step with the context's mixed-precision smoother (FP32 or BF16 storage;
+   FP64 contexts: st_mg_smooth) */
+int st_mg_smooth_mp(st_ctx *c, int level, int color, const double *b, const double *xin, double *xout);
 /* bc = R rf (fine level -> level+1), xf_out = xf + P ec */
 int st_mg_restrict(st_ctx *c, int level, const double *rf, double *bc);
 int st_mg_prolong(st_ctx *c, int level, const double *ec, const double *xf, double *xf_out);

@@ -94,6 +94,7 @@ SIGNATURES = {
     "st_mg_vcycle": (I, [C, D, D]),
     "st_mg_level_desc": (I, [C, I, U8, D]),
     "st_mg_smooth": (I, [C, I, I, D, D, D]),
+    "st_mg_smooth_mp": (I, [C, I, I, D, D, D]),
     "st_mg_restrict": (I, [C, I, D, D]),
     "st_mg_prolong": (I, [C, I, D, D, D]),
 }

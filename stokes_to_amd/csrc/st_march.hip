@@ -1,6 +1,6 @@
 // 2.5D "marching" kernels for the fine 3D levels (gfx950).
 //
-// A workgroup of 4 x 64 threads owns a (j, k) tile of node columns and walks a
+// A workgroup of BJ x 64 threads owns a (j, k) tile of node columns and walks a
 // chunk of i-planes.  For each plane it stages, once, in LDS:
 //   w = m u (masked face velocities), p, flags          3 rolling planes
 //   edge curls e = Cod1 w   (CX/grid3D.cu:469-568)       2 rolling planes
@@ -19,13 +19,7 @@
 namespace st {
 
 namespace {
-// tile shape (j rows x k columns, BJ * BK = 256 threads); BK is a power of two
-#ifndef ST_MARCH_BJ
-#define ST_MARCH_BJ 4
-#endif
-constexpr int BJ = ST_MARCH_BJ, BK = 256 / ST_MARCH_BJ, LJ = BJ + 3, LK = BK + 3, NL = LJ * LK;
-constexpr int NT = BJ * BK;
-static_assert(NL <= 2 * NT, "a plane's staged nodes are fetched two per thread");
+constexpr int BK = 64, LK = BK + 3;
 enum { M_APPLY = 0, M_RESID = 1, M_SMOOTH = 2 };
 
 __device__ __forceinline__ int slot3(int s) { return (s + 3) % 3; }
@@ -36,23 +30,34 @@ __device__ __forceinline__ double wsum64(double v)
     for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
     return v;
 }
+
+// 1/x and a/b of the block solve: IEEE division in FP64; in FP32 (the
+// mixed-precision smoother) the hardware reciprocal, 1 ulp
+__device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
+__device__ __forceinline__ double quo(double a, double b) { return a / b; }
+__device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float quo(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 }  // namespace
 
 // TS: storage of alpha, b and y (and the compute type through Cvt<TS>::C); TX: storage
-// of x (a lower-precision x is widened on load: the FP64 operator on an FP32 vector)
-template <typename TS, int MODE, typename TX = TS>
-__global__ void __launch_bounds__(256) k_march(Lvl g, const uint8_t *__restrict__ fl, const TS *__restrict__ al,
-                                               const TS *__restrict__ b, const TX *__restrict__ x,
-                                               TS *__restrict__ y, double *part, int chunk, int color, float omega)
+// of x (a lower-precision x is widened on load: the FP64 operator on an FP32 vector);
+// BJ: tile rows
+template <typename TS, int MODE, typename TX = TS, int BJ = 4>
+__global__ void __launch_bounds__(BJ * 64) k_march(Lvl g, const uint8_t *__restrict__ fl, const TS *__restrict__ al,
+                                                   const TS *__restrict__ b, const TX *__restrict__ x,
+                                                   TS *__restrict__ y, double *part, int chunk, int color,
+                                                   float omega)
 {
     typedef typename Cvt<TS>::C C;
+    constexpr int LJ = BJ + 3, NL = LJ * LK, NT = BJ * BK;
+    static_assert(NL <= 2 * NT, "a plane's staged nodes are fetched two per thread");
+    constexpr bool SM = (MODE == M_SMOOTH);
     __shared__ C sW[3][3][NL];
     __shared__ C sP[3][NL];
     __shared__ uint8_t sF[3][NL];
     __shared__ C sE[2][3][NL];
     __shared__ C sG[2][NL];
     __shared__ double sred[NT / 64];
-
     const int tid = threadIdx.x;
     const int tk = tid & (BK - 1), tj = tid / BK;
     // XCD-aware tile order: workgroups are dealt round-robin to the 8 XCDs, so map
@@ -65,7 +70,7 @@ __global__ void __launch_bounds__(256) k_march(Lvl g, const uint8_t *__restrict_
     const int bx = lid % nbx, by = (lid / nbx) % nby, bz = lid / (nbx * nby);
     const int k0 = bx * BK, j0 = by * BJ;
     const int j = j0 + tj, k = k0 + tk;
-    const int ilast = (MODE == M_SMOOTH) ? g.nx : g.nx + 1;  // output planes [0, ilast)
+    const int ilast = SM ? g.nx : g.nx + 1;  // output planes [0, ilast)
     const int ic0 = bz * chunk;
     int ic1 = (bz == nbz - 1) ? ilast : ic0 + chunk;
     if (ic1 > ilast) ic1 = ilast;
@@ -154,13 +159,12 @@ __global__ void __launch_bounds__(256) k_march(Lvl g, const uint8_t *__restrict_
     fetch(ic0 + 2);
     __syncthreads();
 
-    // global operands of the node (i, j, k) and of its three high faces, loaded
-    // before the two LDS phases of the iteration so that their latency overlaps them:
-    // pb = b (p, x, y, z rows; then the x/y/z rows of the high faces), px = raw x of
-    // the own faces then of the high faces (the values of fixed faces), pa = alpha
+    // global operands of the node q and of its three high faces, loaded before the
+    // two LDS phases of the iteration so that their latency overlaps them: pb = b
+    // (p, x, y, z rows; then the x/y/z rows of the high faces), px = raw x of the
+    // own faces then of the high faces (the values of fixed faces), pa = alpha
     C pb[7], px[6], pa[6];
-    auto preload = [&](int i) {
-        const long q = (long)i * sI + (long)j * sJ + k;
+    auto preload = [&](long q) {
         if (MODE != M_APPLY) {
             pb[0] = Cvt<TS>::ld(b[q]);
             pb[1] = Cvt<TS>::ld(b[S + q]);
@@ -173,7 +177,7 @@ __global__ void __launch_bounds__(256) k_march(Lvl g, const uint8_t *__restrict_
         pa[0] = Cvt<TS>::ld(al[q]);
         pa[1] = Cvt<TS>::ld(al[S + q]);
         pa[2] = Cvt<TS>::ld(al[2 * S + q]);
-        if (MODE == M_SMOOTH) {  // i < nx, j < ny, k < nz: the high faces exist
+        if (SM) {  // a cell: i < nx, j < ny, k < nz, so the high faces exist
             pb[4] = Cvt<TS>::ld(b[S + q + sI]);
             pb[5] = Cvt<TS>::ld(b[2 * S + q + sJ]);
             pb[6] = Cvt<TS>::ld(b[3 * S + q + 1]);
@@ -190,13 +194,13 @@ __global__ void __launch_bounds__(256) k_march(Lvl g, const uint8_t *__restrict_
     for (int i = ic0; i < ic1; i++) {
         store(i + 2);
         if (i + 1 < ic1) fetch(i + 3);
-        preload(i);
+        const long q = (long)i * sI + (long)j * sJ + k;
+        preload(q);
         __syncthreads();
         edges(i + 1);
         gdiv(i + 1);
         __syncthreads();
         const int ps = slot3(i), pn = slot3(i + 1), es = slot2(i), en = slot2(i + 1), gs = slot2(i), gn = slot2(i + 1);
-        const long q = (long)i * sI + (long)j * sJ + k;
 
         // operator row values at plane i, LDS position c; xr / av are the raw x and
         // the alpha of that face (used for a fixed face / the Brinkman term)
@@ -217,7 +221,7 @@ __global__ void __launch_bounds__(256) k_march(Lvl g, const uint8_t *__restrict_
             return (sF[ps][c] & 8) ? r : xr;
         };
 
-        if (MODE != M_SMOOTH) {
+        if (!SM) {
             auto emit = [&](int c, long qq, C gim1, C pim1, const C *bb, const C *xr, const C *av) {
                 C v0 = yp(c), v1 = yx(c, gim1, pim1, xr[0], av[0]), v2 = yy(c, xr[1], av[1]), v3 = yz(c, xr[2], av[2]);
                 if (MODE == M_RESID) {
@@ -265,6 +269,8 @@ __global__ void __launch_bounds__(256) k_march(Lvl g, const uint8_t *__restrict_
             const bool mine = ((i + j + k) & 1) == color;
             const int fb = sF[ps][c];
             if (!mine) {
+                // copy-through: p, and the domain-boundary faces (no cell of the
+                // colour owns them)
                 y[q] = Cvt<TS>::st(sP[ps][c]);
                 if (i == 0) y[S + q] = Cvt<TS>::st(px[0]);
                 if (i == g.nx - 1) y[S + q + sI] = Cvt<TS>::st(px[3]);
@@ -309,27 +315,26 @@ __global__ void __launch_bounds__(256) k_march(Lvl g, const uint8_t *__restrict_
                     a1[d] = m1[d] * (mc + chi[d] + C(4)) + al1 + (C(1) - m1[d]);
                 }
                 C Sc = C(1) - mc, tp = rp;
-                C of[3];
+                C of[3], id[3];
 #pragma unroll
                 for (int d = 0; d < 3; d++) {
                     of[d] = -m0[d] * mc * m1[d];
-                    const C id = C(1) / (a0[d] * a1[d] - of[d] * of[d]);
-                    const C kb0 = id * (a1[d] * (-m0[d]) - of[d] * m1[d]);
-                    const C kb1 = id * (-of[d] * (-m0[d]) + a0[d] * m1[d]);
-                    const C kr0 = id * (a1[d] * r0[d] - of[d] * r1[d]);
-                    const C kr1 = id * (-of[d] * r0[d] + a0[d] * r1[d]);
+                    id[d] = rcp(a0[d] * a1[d] - of[d] * of[d]);
+                    const C kb0 = id[d] * (a1[d] * (-m0[d]) - of[d] * m1[d]);
+                    const C kb1 = id[d] * (-of[d] * (-m0[d]) + a0[d] * m1[d]);
+                    const C kr0 = id[d] * (a1[d] * r0[d] - of[d] * r1[d]);
+                    const C kr1 = id[d] * (-of[d] * r0[d] + a0[d] * r1[d]);
                     Sc -= mc * ((-m0[d]) * kb0 + m1[d] * kb1);
                     tp -= mc * ((-m0[d]) * kr0 + m1[d] * kr1);
                 }
-                const C dp = (Sc != C(0)) ? tp / Sc : C(0);
+                const C dp = (Sc != C(0)) ? quo(tp, Sc) : C(0);
                 const C om = (C)omega;
                 y[q] = Cvt<TS>::st(sP[ps][c] + om * dp);
 #pragma unroll
                 for (int d = 0; d < 3; d++) {
-                    const C id = C(1) / (a0[d] * a1[d] - of[d] * of[d]);
                     const C e0 = r0[d] + m0[d] * dp, e1 = r1[d] - m1[d] * dp;
-                    y[(long)(1 + d) * S + q] = Cvt<TS>::st(u0[d] + om * id * (a1[d] * e0 - of[d] * e1));
-                    y[(long)(1 + d) * S + qh[d]] = Cvt<TS>::st(u1[d] + om * id * (-of[d] * e0 + a0[d] * e1));
+                    y[(long)(1 + d) * S + q] = Cvt<TS>::st(u0[d] + om * id[d] * (a1[d] * e0 - of[d] * e1));
+                    y[(long)(1 + d) * S + qh[d]] = Cvt<TS>::st(u1[d] + om * id[d] * (-of[d] * e0 + a0[d] * e1));
                 }
             }
         }
@@ -350,24 +355,26 @@ __global__ void __launch_bounds__(256) k_march(Lvl g, const uint8_t *__restrict_
     }
 }
 
-bool march_ok(const Lvl &g) { return g.dim == 3 && g.nz >= 64 && g.nz % BK == 0 && g.ny % BJ == 0 && g.nx >= 8; }
+bool march_ok(const Lvl &g) { return g.dim == 3 && g.nz >= 64 && g.nz % BK == 0 && g.ny % 4 == 0 && g.nx >= 8; }
 
-static dim3 march_grid(const Lvl &g, int &chunk)
+// grid of BJ-row tiles x i-chunks: at least `target` workgroups, at least 8 planes
+// per chunk
+static dim3 march_grid(const Lvl &g, int bj, int target, int &chunk)
 {
-    const int tiles = (g.nz / BK) * (g.ny / BJ);
-    int nch = (2048 + tiles - 1) / tiles;      // aim at >= 2048 workgroups
-    int maxch = g.nx / 8;                      // at least 8 planes per chunk
+    const int tiles = (g.nz / BK) * (g.ny / bj);
+    int nch = (target + tiles - 1) / tiles;
+    int maxch = g.nx / 8;
     if (nch > maxch) nch = maxch;
     if (nch < 1) nch = 1;
     chunk = (g.nx + nch - 1) / nch;
     nch = (g.nx + chunk - 1) / chunk;
-    return dim3(g.nz / BK, g.ny / BJ, nch);
+    return dim3(g.nz / BK, g.ny / bj, nch);
 }
 
 unsigned march_blocks(const Lvl &g)
 {
     int ch;
-    dim3 d = march_grid(g, ch);
+    dim3 d = march_grid(g, 4, 2048, ch);
     return d.x * d.y * d.z;
 }
 
@@ -375,8 +382,8 @@ template <typename TS>
 void march_apply(const Lvl &g, const uint8_t *fl, const TS *al, const TS *x, TS *y, hipStream_t s)
 {
     int ch;
-    dim3 grid = march_grid(g, ch);
-    hipLaunchKernelGGL((k_march<TS, M_APPLY>), grid, dim3(NT), 0, s, g, fl, al, (const TS *)nullptr, x, y,
+    dim3 grid = march_grid(g, 4, 2048, ch);
+    hipLaunchKernelGGL((k_march<TS, M_APPLY>), grid, dim3(256), 0, s, g, fl, al, (const TS *)nullptr, x, y,
                        (double *)nullptr, ch, 0, 0.f);
 }
 template <typename TS>
@@ -384,25 +391,25 @@ void march_residual(const Lvl &g, const uint8_t *fl, const TS *al, const TS *b, 
                     hipStream_t s)
 {
     int ch;
-    dim3 grid = march_grid(g, ch);
-    hipLaunchKernelGGL((k_march<TS, M_RESID>), grid, dim3(NT), 0, s, g, fl, al, b, x, r, part, ch, 0, 0.f);
+    dim3 grid = march_grid(g, 4, 2048, ch);
+    hipLaunchKernelGGL((k_march<TS, M_RESID>), grid, dim3(256), 0, s, g, fl, al, b, x, r, part, ch, 0, 0.f);
 }
 template <typename TS>
 void march_smooth(const Lvl &g, const uint8_t *fl, const TS *al, const TS *b, const TS *xin, TS *xout, int color,
                   float omega, hipStream_t s)
 {
     int ch;
-    dim3 grid = march_grid(g, ch);
-    hipLaunchKernelGGL((k_march<TS, M_SMOOTH>), grid, dim3(NT), 0, s, g, fl, al, b, xin, xout, (double *)nullptr,
+    dim3 grid = march_grid(g, 4, 2048, ch);
+    hipLaunchKernelGGL((k_march<TS, M_SMOOTH>), grid, dim3(256), 0, s, g, fl, al, b, xin, xout, (double *)nullptr,
                        ch, color, omega);
 }
 
 void march_apply_mixed(const Lvl &g, const uint8_t *fl, const double *al, const float *x, double *y, hipStream_t s)
 {
     int ch;
-    dim3 grid = march_grid(g, ch);
-    hipLaunchKernelGGL((k_march<double, M_APPLY, float>), grid, dim3(NT), 0, s, g, fl, al, (const double *)nullptr, x,
-                       y, (double *)nullptr, ch, 0, 0.f);
+    dim3 grid = march_grid(g, 4, 2048, ch);
+    hipLaunchKernelGGL((k_march<double, M_APPLY, float>), grid, dim3(256), 0, s, g, fl, al, (const double *)nullptr,
+                       x, y, (double *)nullptr, ch, 0, 0.f);
 }
 
 #define INST_M(T)                                                                                              \

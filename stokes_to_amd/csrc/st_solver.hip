@@ -539,6 +539,23 @@ static int check_finite(st_ctx *c, const double *h, long n, const char *what)
     return ST_OK;
 }
 
+// one smoother colour step in FP32 / BF16 storage (test entry st_mg_smooth_mp)
+template <typename TS> static int mg_smooth_lowp(st_ctx *c, int l, int color, const double *b, const double *xin,
+                                                 double *xout)
+{
+    const Lvl &g = c->lv[l];
+    const long nd = abi_dofs(c->dim, g.nx);
+    TS *bb = reinterpret_cast<TS *>(c->BA), *xi = reinterpret_cast<TS *>(c->Wv), *xo = reinterpret_cast<TS *>(c->LAM);
+    CKR(stage_in(c, b, nd));
+    launch_pack<TS>(g, c->abi_a, bb, 0, c->fl[l], 1.0, c->s);
+    CKR(stage_in(c, xin, nd));
+    launch_pack<TS>(g, c->abi_a, xi, 0, c->fl[l], 1.0, c->s);
+    launch_fill<TS>((long)g.nfield * g.S, xo, 0.0, c->s);
+    launch_smooth<TS>(g, c->fl[l], hier<TS>(c).al[l], bb, xi, xo, color, (float)c->prm.omega, false, c->s);
+    launch_unpack<TS>(g, xo, c->abi_b, c->s);
+    return stage_out(c, xout, nd);
+}
+
 extern "C" {
 
 void st_default_params(st_params *p, int dim, int n)
@@ -1436,6 +1453,15 @@ int st_mg_smooth(st_ctx *c, int l, int color, const double *b, const double *xin
     launch_smooth<double>(g, c->fl[l], c->h64.al[l], c->BA, c->Wv, c->LAM, color, (float)c->prm.omega, false, c->s);
     launch_unpack<double>(g, c->LAM, c->abi_b, c->s);
     return stage_out(c, xout, nd);
+}
+
+int st_mg_smooth_mp(st_ctx *c, int l, int color, const double *b, const double *xin, double *xout)
+{
+    if (!c || l < 0 || l >= c->L || !b || !xin || !xout) return ST_EINVAL;
+    if (c->dirty) CKR(setup_hierarchy(c));
+    if (c->prm.precision == ST_FP32) return mg_smooth_lowp<float>(c, l, color, b, xin, xout);
+    if (c->prm.precision == ST_BF16) return mg_smooth_lowp<bf16>(c, l, color, b, xin, xout);
+    return st_mg_smooth(c, l, color, b, xin, xout);
 }
 
 int st_mg_restrict(st_ctx *c, int l, const double *rf, double *bc)

@@ -247,10 +247,13 @@ class FluidSteadyState:
         self._check(self.lib.st_mg_vcycle(self.h, dptr(_f64(b)), dptr(z)))
         return z
 
-    def mg_smooth(self, l, color, b, xin):
+    def mg_smooth(self, l, color, b, xin, mixed=False):
+        """One smoother colour step on level l; mixed=True runs the context's
+        mixed-precision (FP32/BF16 storage) smoother."""
         nd = dof_sizes(self.dim, self.mg_level_n(l))[-1]
         out = np.zeros(nd)
-        self._check(self.lib.st_mg_smooth(self.h, l, color, dptr(_f64(b)), dptr(_f64(xin)), dptr(out)))
+        fn = self.lib.st_mg_smooth_mp if mixed else self.lib.st_mg_smooth
+        self._check(fn(self.h, l, color, dptr(_f64(b)), dptr(_f64(xin)), dptr(out)))
         return out
 
     def mg_restrict(self, l, rf):

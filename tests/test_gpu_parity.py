@@ -116,8 +116,10 @@ def test_mg_components_match_oracle(make_solver, orc, dim, n):
             assert rel(s.mg_prolong(l, ec, xin), xo) < 1e-13
 
 
-@pytest.mark.parametrize("dim,n", [(3, 16), (2, 32)])
+@pytest.mark.parametrize("dim,n", [(3, 16), (2, 32), (3, 64)])
 def test_vcycle_matches_oracle(make_solver, orc, dim, n):
+    """At 64^3 the fine level runs the tiled kernels, including the smoother pass from
+    x = 0 that starts the V-cycle."""
     P, rho = setup(dim, n)
     s = solver_for(make_solver, P, rho)
     fixed, vol, pen = oracle_desc(orc, P, rho)
@@ -170,6 +172,27 @@ def test_fine_level_tiled_kernels_match_oracle(make_solver, orc, dim, n):
     for color in (0, 1):
         xo = orc.smooth_color(dim, P.shape, fixed, pen, b, x, color, 1.0)
         assert rel(s.mg_smooth(0, color, b, x), xo) < 1e-12
+
+
+@pytest.mark.parametrize("prec,tol", [("fp32", 1e-6), ("bf16", 1e-2)])
+def test_mixed_precision_smoother_matches_oracle(make_solver, orc, prec, tol):
+    """The FP32 / BF16 smoother colour step against the FP64 oracle: level 0 of 64^3
+    runs the tiled (marching) kernel, level 1 (32^3) the per-cell kernel.  The bar is
+    the storage rounding of b, x and alpha (measured 2e-7 in FP32, 2.2e-3 in BF16)."""
+    dim, n = 3, 64
+    P, rho = setup(dim, n)
+    s = solver_for(make_solver, P, rho, precision=prec)
+    fixed, vol, pen = oracle_desc(orc, P, rho)
+    mg = orc.MG(dim, P.shape, fixed, pen, nu_pre=2, nu_post=2, omega=1.0, tang=0)
+    rng = np.random.default_rng(20)
+    for l in (0, 1):
+        fo, po = mg.level_desc(l)
+        nl = mg.level_dims(l)[0]
+        b = rng.standard_normal(fo.size)
+        x = rng.standard_normal(fo.size)
+        for color in (0, 1):
+            xo = orc.smooth_color(dim, (nl,) * dim, fo, po, b, x, color, 1.0)
+            assert rel(s.mg_smooth(l, color, b, x, mixed=True), xo) < tol, (l, color)
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32", "bf16"])
