@@ -42,10 +42,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# measured HBM traffic per launch of the 256^3 FP64 smoother colour pass: rocprofv3
+# measured HBM traffic per launch of the 256^3 smoother colour pass: rocprofv3
 # FETCH_SIZE and WRITE_SIZE runs scaled by a 1 GiB read / write of known size
 # (profiles/k_march_smooth256_counters.md)
-SMOOTH256_FP64_TRAFFIC_BYTES = 2.94e9
+SMOOTH256_TRAFFIC_BYTES = {"fp64": 2.761e9, "fp32": 1.472e9}
 METRIC = "ms per Stokes solve to 1e-8 rel residual at 3D 256³: plain FP64 MG vs learned+MP"
 
 
@@ -105,15 +105,18 @@ def summary(stats):
     }
 
 
-def roofline_of(prof, n):
+def roofline_of(prof, n, prec):
+    """Roofline of the finest-level smoother colour pass (`k_march<prec, SMOOTH>`) from
+    the solver's own HIP-event launch timings."""
     pr = prof["smooth"]
     avg_ms = pr["total_ms"] / max(pr["launches"], 1)
     achieved = pr["bytes"] / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    ctype = {"fp64": "double", "fp32": "float", "bf16": "bf16"}[prec]
     return {
-        "bound": "hbm", "kernel": "k_march<double,SMOOTH> (256^3 finest-level colour pass, plain FP64 path)",
+        "bound": "hbm", "kernel": f"k_march<{ctype},SMOOTH> ({n}^3 finest-level colour pass, {prec} storage)",
         "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": SMOOTH256_FP64_TRAFFIC_BYTES if n == 256 else None,
+        "traffic": SMOOTH256_TRAFFIC_BYTES.get(prec) if n == 256 else None,
         "avg_launch_ms": round(avg_ms, 4), "launches": pr["launches"],
         "algorithmic_bytes_per_launch": pr["bytes"],
     }
@@ -135,6 +138,7 @@ def run_to(args, prob):
         if it == args.warmup:
             torch.cuda.synchronize()
             Z.lib.st_profile(Z.h, 1)
+            K.lib.st_profile(K.h, 1)   # the warm-start context runs the same MP kernels as L
             t_region = time.perf_counter()
         rho = L.get_design()
         rhof = L.filter(rho)
@@ -165,11 +169,13 @@ def run_to(args, prob):
     torch.cuda.synchronize()
     region_ms = (time.perf_counter() - t_region) * 1e3 if t_region is not None else 0.0
     Z.lib.st_profile(Z.h, 0)
+    K.lib.st_profile(K.h, 0)
     prof = read_profile(Z)
+    prof_mp = read_profile(K)
     last_rhof = L.filter(L.get_design())
     extra = {"train_ms_per_iter": train_ms, "train_ms_total": float(sum(train_ms)),
              "inference_ms_median": float(np.median(inf_ms)) if inf_ms else 0.0,
-             "learned_taken": int(sum(used)), "prof": prof, "last_design": last_rhof,
+             "learned_taken": int(sum(used)), "prof": prof, "prof_mp": prof_mp, "last_design": last_rhof,
              "loss_vs_zero_median": float(np.median(lossr)) if lossr else None,
              "final_J": r["J"], "final_vol": r["vol"], "region_ms": region_ms}
     for s in (L, Z, K):
@@ -323,7 +329,9 @@ def main():
             "timed_region_wall_ms": round(extra["region_ms"], 1),
             "timed_region_note": "wall time of the timed iterations, each solving the design three times "
                                  "(plain, warm start, learned) plus the adjoint, OC and host copies",
-            "roofline": roofline_of(extra["prof"], args.n), "kernels_plain_fp64": extra["prof"],
+            "roofline": roofline_of(extra["prof_mp"], args.n, args.mp),
+            "roofline_plain_fp64": roofline_of(extra["prof"], args.n, "fp64"),
+            "kernels_plain_fp64": extra["prof"], f"kernels_{args.mp}": extra["prof_mp"],
         })
         base_design, base_iters = extra["last_design"], int(np.median(plain["iters"]))
         log(args, f"[bench] plain {plain['median_ms']:.1f} ms, keep+{args.mp} {keep['median_ms']:.1f} ms, "
@@ -339,7 +347,8 @@ def main():
                                    f"{args.mean}, seed 12345), rtol {args.rtol} true FP64 residual",
                        "n": args.n, "dofs": int(prob.fixed.size), "rtol": args.rtol, "mp": args.mp},
             "plain_fp64": p, "mixed": m, "speedup_vs_plain": round(p["median_ms"] / value, 3),
-            "roofline": roofline_of(pp, args.n), "kernels": {"fp64": pp, args.mp: mp},
+            "roofline": roofline_of(mp, args.n, args.mp), "roofline_plain_fp64": roofline_of(pp, args.n, "fp64"),
+            "kernels": {"fp64": pp, args.mp: mp},
         })
         base_design, base_iters = rho, int(np.median(p["iters"]))
         log(args, f"[bench] fp64 {p['median_ms']:.1f} ms, {args.mp} {m['median_ms']:.1f} ms")

@@ -798,16 +798,20 @@ int st_profile_read(st_ctx *c, int kind, double *total_ms, long *launches, doubl
     if (bytes) {
         // algorithmic HBM bytes of one finest-level launch over the valid DOFs, with the
         // masks counted as bit fields (n_f/8 bytes):
-        //   operator apply      s(2 n_c + 3 n_f) + n_f/8   (x read, alpha read, y write)
+        //   operator apply      (s_x + s_y)(n_c + n_f) + s_a n_f + n_f/8  (x, alpha read, y write)
         //   residual            s(3 n_c + 4 n_f) + n_f/8   (x, b, alpha read, r write)
         //   smoother colour     half of a two-colour sweep s(3 n_c + 4 n_f) + n_f/8
         //                       (x, b, alpha read once, x written once per sweep)
+        // s: storage of the level-0 multigrid vectors; the Krylov operator of the FP32
+        // path is FP32, that of the BF16 path FP64 on FP32 vectors
         const double nc = (double)c->ncell, nf = (double)(c->ndof - c->ncell);
-        double sv = 8.0;
-        if (kind != ST_PROF_APPLY) sv = (c->prm.precision == ST_FP32) ? 4.0 : (c->prm.precision == ST_BF16 ? 2.0 : 8.0);
+        const int pr = c->prm.precision;
+        const double sv = (pr == ST_FP32) ? 4.0 : (pr == ST_BF16 ? 2.0 : 8.0);
         if (kind == ST_PROF_SMOOTH) *bytes = 0.5 * (sv * (3.0 * nc + 4.0 * nf) + nf / 8.0);
-        else if (kind == ST_PROF_APPLY) *bytes = sv * (2.0 * nc + 3.0 * nf) + nf / 8.0;
-        else *bytes = sv * (3.0 * nc + 4.0 * nf) + nf / 8.0;
+        else if (kind == ST_PROF_APPLY) {
+            const double sx = (pr == ST_FP64) ? 8.0 : 4.0, sa = (pr == ST_FP32) ? 4.0 : 8.0, sy = sa;
+            *bytes = (sx + sy) * (nc + nf) + sa * nf + nf / 8.0;
+        } else *bytes = sv * (3.0 * nc + 4.0 * nf) + nf / 8.0;
     }
     return ST_OK;
 }
