@@ -85,7 +85,8 @@ template <typename TA, typename TB> void launch_dot(long n, const TA *a, const T
 template <typename TW, typename TV>
 void launch_multidot(long n, const TW *w, const TV *V, long ld, int m, double *part, hipStream_t);
 template <typename TW, typename TV>
-void launch_multiaxpy(long n, TW *w, const TV *V, long ld, int m, const double *h, double *part, hipStream_t);
+void launch_multiaxpy(long n, TW *w, const TV *V, long ld, int m, const double *h, const double *sc, double *part,
+                      hipStream_t);
 template <typename TX, typename TY> void launch_axpby(long n, double a, const TX *x, double b, TY *y, hipStream_t);
 template <typename TS> void launch_fill(long n, TS *y, double v, hipStream_t);
 

@@ -1050,14 +1050,14 @@ __global__ void __launch_bounds__(256) k_multidot(long n, const TW *w, const TV 
     }
 }
 
-// w -= sum_v h[v] V[v]; partial sums of |w|^2
+// w -= sum_v h[v] sc[v] V[v] (sc optional); partial sums of |w|^2
 template <typename TW, typename TV, int M>
 __global__ void __launch_bounds__(256) k_multiaxpy(long n, TW *w, const TV *V, long ld, int m, const double *h,
-                                                   double *part)
+                                                   const double *sc, double *part)
 {
     __shared__ double sh[16];
     __shared__ double hs[64];
-    if (threadIdx.x < (unsigned)m) hs[threadIdx.x] = h[threadIdx.x];
+    if (threadIdx.x < (unsigned)m) hs[threadIdx.x] = sc ? h[threadIdx.x] * sc[threadIdx.x] : h[threadIdx.x];
     __syncthreads();
     double acc = 0.0;
     for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
@@ -1259,11 +1259,12 @@ void launch_multidot(long n, const TW *w, const TV *V, long ld, int m, double *p
     else hipLaunchKernelGGL((k_multidot<TW, TV, 32>), dim3(nb_grid(n)), dim3(256), 0, s, n, w, V, ld, m, part);
 }
 template <typename TW, typename TV>
-void launch_multiaxpy(long n, TW *w, const TV *V, long ld, int m, const double *h, double *part, hipStream_t s)
+void launch_multiaxpy(long n, TW *w, const TV *V, long ld, int m, const double *h, const double *sc, double *part,
+                      hipStream_t s)
 {
-    if (m <= 8) hipLaunchKernelGGL((k_multiaxpy<TW, TV, 8>), dim3(nb_grid(n)), dim3(256), 0, s, n, w, V, ld, m, h, part);
-    else if (m <= 16) hipLaunchKernelGGL((k_multiaxpy<TW, TV, 16>), dim3(nb_grid(n)), dim3(256), 0, s, n, w, V, ld, m, h, part);
-    else hipLaunchKernelGGL((k_multiaxpy<TW, TV, 32>), dim3(nb_grid(n)), dim3(256), 0, s, n, w, V, ld, m, h, part);
+    if (m <= 8) hipLaunchKernelGGL((k_multiaxpy<TW, TV, 8>), dim3(nb_grid(n)), dim3(256), 0, s, n, w, V, ld, m, h, sc, part);
+    else if (m <= 16) hipLaunchKernelGGL((k_multiaxpy<TW, TV, 16>), dim3(nb_grid(n)), dim3(256), 0, s, n, w, V, ld, m, h, sc, part);
+    else hipLaunchKernelGGL((k_multiaxpy<TW, TV, 32>), dim3(nb_grid(n)), dim3(256), 0, s, n, w, V, ld, m, h, sc, part);
 }
 template <typename TX, typename TY>
 void launch_axpby(long n, double a, const TX *x, double b, TY *y, hipStream_t s)
@@ -1327,7 +1328,8 @@ INST_PAIR(float, bf16)
 
 #define INST_KRYLOV(W, V)                                                                                   \
     template void launch_multidot<W, V>(long, const W *, const V *, long, int, double *, hipStream_t);        \
-    template void launch_multiaxpy<W, V>(long, W *, const V *, long, int, const double *, double *, hipStream_t);
+    template void launch_multiaxpy<W, V>(long, W *, const V *, long, int, const double *, const double *, double *, \
+                                         hipStream_t);
 INST_KRYLOV(double, double)
 INST_KRYLOV(double, float)
 INST_KRYLOV(float, float)

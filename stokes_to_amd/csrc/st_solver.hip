@@ -53,7 +53,7 @@ struct st_ctx {
     long ndof = 0, ncell = 0, nvec = 0;  // ABI sizes and padded vector length
     double *X = nullptr, *B = nullptr, *XD = nullptr, *BODY = nullptr, *LAM = nullptr, *BA = nullptr, *ZERO = nullptr;
     double *Wv = nullptr, *V = nullptr, *Zb = nullptr;
-    double *part = nullptr, *red = nullptr, *h_dev = nullptr;
+    double *part = nullptr, *red = nullptr, *h_dev = nullptr, *sig2_dev = nullptr;
     int npart = 0;
     // design
     double *rho = nullptr, *rhof = nullptr, *dj = nullptr, *dc = nullptr, *rn = nullptr, *fW = nullptr, *dv = nullptr;
@@ -428,6 +428,13 @@ static void inner_apply(st_ctx *c, const float *z, double *w) { apply64(c, z, w)
 // restart takes over.
 template <typename TV, typename TW> static int fgmres(st_ctx *c, const double *b, double *x, st_stats *st)
 {
+    // LAZY (Krylov vectors and operator in one precision): the basis is stored
+    // unnormalised, v_i = sig_i u_i, and A M u_j is written straight into the slot of
+    // u_{j+1}, so no pass scales and copies the Arnoldi vector.  With c_i = u_i . A M u_j:
+    // h_ij = sig_i sig_j c_i, u_{j+1} = A M u_j - sum sig_i^2 c_i u_i,
+    // h_{j+1,j} = sig_j |u_{j+1}|, sig_{j+1} = 1 / |u_{j+1}|; z_j = M u_j enters the
+    // solution update with the weight sig_j.
+    constexpr bool LAZY = std::is_same<TV, TW>::value;
     TV *const Vb = reinterpret_cast<TV *>(c->V);
     TV *const Zs = reinterpret_cast<TV *>(c->Zb);
     TW *const Wk = reinterpret_cast<TW *>(c->Wv);
@@ -442,6 +449,7 @@ template <typename TV, typename TW> static int fgmres(st_ctx *c, const double *b
     if (!std::isfinite(bn)) return fail(c, ST_ENAN, "non-finite right-hand side");
     if (bn == 0.0) bn = 1.0;
     std::vector<double> H((size_t)(m + 1) * m, 0.0), cs(m), sn(m), gv(m + 1), y(m), hh(m + 2);
+    std::vector<double> sig(m + 1, 1.0), sig2(m + 1, 1.0);
     int it = 0;
     double rr = 0.0, rr0 = -1.0;
     const int rb = (int)residual_blocks(g);
@@ -457,29 +465,48 @@ template <typename TV, typename TW> static int fgmres(st_ctx *c, const double *b
         rr = beta / bn;
         if (rr0 < 0) rr0 = rr;
         if (rr <= rtol || it >= c->prm.max_iter) break;
-        launch_axpby<double, TV>(nv, 1.0 / beta, r0, 0.0, Vb, c->s);
+        if (LAZY) {
+            sig[0] = 1.0 / beta;
+            sig2[0] = sig[0] * sig[0];
+            CK(hipMemcpyAsync(c->sig2_dev, sig2.data(), sizeof(double), hipMemcpyHostToDevice, c->s));
+            if (!std::is_same<TV, double>::value) launch_axpby<double, TV>(nv, 1.0, r0, 0.0, Vb, c->s);
+        } else {
+            launch_axpby<double, TV>(nv, 1.0 / beta, r0, 0.0, Vb, c->s);
+        }
         std::fill(gv.begin(), gv.end(), 0.0);
         gv[0] = beta;
         const double cycle_tol = std::max(rtol * bn, floor_rel * beta);
         int j = 0;
         for (; j < m && it < c->prm.max_iter; j++) {
             TV *vj = Vb + (long)j * nv, *zj = Zs + (long)j * nv;
+            TW *w = LAZY ? reinterpret_cast<TW *>(Vb + (long)(j + 1) * nv) : Wk;
             precondition(c, vj, zj);
             prof_mark(c, ST_PROF_APPLY);
-            inner_apply(c, zj, Wk);
+            inner_apply(c, zj, w);
             prof_mark(c, ST_PROF_APPLY);
             const unsigned nbk = blas_blocks(nv);
-            launch_multidot<TW, TV>(nv, Wk, Vb, nv, j + 1, c->part, c->s);
+            launch_multidot<TW, TV>(nv, w, Vb, nv, j + 1, c->part, c->s);
             launch_sum((int)nbk, j + 1, c->part, c->h_dev, c->s);
-            launch_multiaxpy<TW, TV>(nv, Wk, Vb, nv, j + 1, c->h_dev, c->part, c->s);
+            launch_multiaxpy<TW, TV>(nv, w, Vb, nv, j + 1, c->h_dev, LAZY ? c->sig2_dev : nullptr, c->part, c->s);
             launch_sum((int)nbk, 1, c->part, c->h_dev + (j + 1), c->s);
             CK(hipMemcpyAsync(hh.data(), c->h_dev, sizeof(double) * (j + 2), hipMemcpyDeviceToHost, c->s));
             CK(hipStreamSynchronize(c->s));
-            const double hn = std::sqrt(hh[j + 1]);
-            if (!std::isfinite(hn)) return fail(c, ST_ENAN, "non-finite Arnoldi vector");
-            for (int i = 0; i <= j; i++) H[(size_t)i * m + j] = hh[i];
+            const double un = std::sqrt(hh[j + 1]);  // |u_{j+1}| (LAZY) or |w|
+            if (!std::isfinite(un)) return fail(c, ST_ENAN, "non-finite Arnoldi vector");
+            const double hn = LAZY ? sig[j] * un : un;
+            for (int i = 0; i <= j; i++) H[(size_t)i * m + j] = LAZY ? sig[i] * sig[j] * hh[i] : hh[i];
             H[(size_t)(j + 1) * m + j] = hn;
-            if (hn > 0) launch_axpby<TW, TV>(nv, 1.0 / hn, Wk, 0.0, Vb + (long)(j + 1) * nv, c->s);
+            if (LAZY) {
+                sig[j + 1] = un > 0 ? 1.0 / un : 0.0;
+                if (un > 1e12 || (un > 0 && un < 1e-12)) {  // keep the stored vectors in range
+                    launch_axpby<TW, TV>(nv, 1.0 / un, w, 0.0, reinterpret_cast<TV *>(w), c->s);
+                    sig[j + 1] = 1.0;
+                }
+                sig2[j + 1] = sig[j + 1] * sig[j + 1];
+                CK(hipMemcpyAsync(c->sig2_dev + (j + 1), &sig2[j + 1], sizeof(double), hipMemcpyHostToDevice, c->s));
+            } else if (hn > 0) {
+                launch_axpby<TW, TV>(nv, 1.0 / hn, Wk, 0.0, Vb + (long)(j + 1) * nv, c->s);
+            }
             for (int i = 0; i < j; i++) {
                 double a = H[(size_t)i * m + j], bb = H[(size_t)(i + 1) * m + j];
                 H[(size_t)i * m + j] = cs[i] * a + sn[i] * bb;
@@ -504,9 +531,9 @@ template <typename TV, typename TW> static int fgmres(st_ctx *c, const double *b
             for (int k = i + 1; k < j; k++) sacc -= H[(size_t)i * m + k] * y[k];
             y[i] = sacc / H[(size_t)i * m + i];
         }
-        for (int i = 0; i < j; i++) y[i] = -y[i];
+        for (int i = 0; i < j; i++) y[i] = LAZY ? -y[i] * sig[i] : -y[i];
         CK(hipMemcpyAsync(c->h_dev, y.data(), sizeof(double) * j, hipMemcpyHostToDevice, c->s));
-        launch_multiaxpy<double, TV>(nv, x, Zs, nv, j, c->h_dev, nullptr, c->s);
+        launch_multiaxpy<double, TV>(nv, x, Zs, nv, j, c->h_dev, nullptr, nullptr, c->s);
     }
     if (st) {
         st->iters = it;
@@ -649,6 +676,7 @@ int st_create(st_ctx **out, const st_params *p)
     if (rc == ST_OK) rc = dalloc(c, &c->part, c->npart);
     if (rc == ST_OK) rc = dalloc(c, &c->red, 64);
     if (rc == ST_OK) rc = dalloc(c, &c->h_dev, 64);
+    if (rc == ST_OK) rc = dalloc(c, &c->sig2_dev, 64);
     for (double **pp : {&c->rho, &c->rhof, &c->dj, &c->dc, &c->rn, &c->fW, &c->dv, &c->tmpc})
         if (rc == ST_OK) rc = dalloc(c, pp, c->ncell);
     if (rc == ST_OK) rc = dalloc(c, &c->abi_a, c->ndof);
