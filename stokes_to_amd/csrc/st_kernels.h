@@ -100,6 +100,9 @@ void march_residual(const Lvl &, const uint8_t *, const TS *, const TS *b, const
 template <typename TS>
 void march_smooth(const Lvl &, const uint8_t *, const TS *, const TS *b, const TS *xin, TS *xout, int color,
                   float omega, hipStream_t);
+// FP64 residual r = b - A x stored in FP32 (+ FP64 partial sums of |r|^2)
+void march_residual_f32(const Lvl &, const uint8_t *, const double *al, const double *b, const double *x, float *r,
+                        double *part, hipStream_t);
 // FP64 operator applied to an FP32-stored vector (x float, alpha and y double)
 void march_apply_mixed(const Lvl &, const uint8_t *, const double *al, const float *x, double *y, hipStream_t);
 

@@ -39,13 +39,13 @@ __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x);
 __device__ __forceinline__ float quo(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 }  // namespace
 
-// TS: storage of alpha, b and y (and the compute type through Cvt<TS>::C); TX: storage
+// TS: storage of alpha and b (and the compute type through Cvt<TS>::C); TX: storage
 // of x (a lower-precision x is widened on load: the FP64 operator on an FP32 vector);
-// BJ: tile rows
-template <typename TS, int MODE, typename TX = TS, int BJ = 4>
+// BJ: tile rows; TY: storage of y (an FP64 residual stored in FP32)
+template <typename TS, int MODE, typename TX = TS, int BJ = 4, typename TY = TS>
 __global__ void __launch_bounds__(BJ * 64) k_march(Lvl g, const uint8_t *__restrict__ fl, const TS *__restrict__ al,
                                                    const TS *__restrict__ b, const TX *__restrict__ x,
-                                                   TS *__restrict__ y, double *part, int chunk, int color,
+                                                   TY *__restrict__ y, double *part, int chunk, int color,
                                                    float omega)
 {
     typedef typename Cvt<TS>::C C;
@@ -231,10 +231,10 @@ __global__ void __launch_bounds__(BJ * 64) k_march(Lvl g, const uint8_t *__restr
                     v3 = bb[3] - v3;
                     acc += (double)v0 * v0 + (double)v1 * v1 + (double)v2 * v2 + (double)v3 * v3;
                 }
-                y[qq] = Cvt<TS>::st(v0);
-                y[S + qq] = Cvt<TS>::st(v1);
-                y[2 * S + qq] = Cvt<TS>::st(v2);
-                y[3 * S + qq] = Cvt<TS>::st(v3);
+                y[qq] = Cvt<TY>::st((typename Cvt<TY>::C)(v0));
+                y[S + qq] = Cvt<TY>::st((typename Cvt<TY>::C)(v1));
+                y[2 * S + qq] = Cvt<TY>::st((typename Cvt<TY>::C)(v2));
+                y[3 * S + qq] = Cvt<TY>::st((typename Cvt<TY>::C)(v3));
             };
             emit(cown, q, gm1, pm1, pb, px, pa);
             // the padded node column k = nz and row j = ny belong to the last tiles
@@ -259,8 +259,8 @@ __global__ void __launch_bounds__(BJ * 64) k_march(Lvl g, const uint8_t *__restr
                 const long qp = (long)i * sI + (long)j * sJ + (g.nz + 1 + tk);
 #pragma unroll
                 for (int f = 0; f < 4; f++) {
-                    y[(long)f * S + qp] = Cvt<TS>::st(C(0));
-                    if (lastj) y[(long)f * S + qp + sJ] = Cvt<TS>::st(C(0));
+                    y[(long)f * S + qp] = Cvt<TY>::st((typename Cvt<TY>::C)(C(0)));
+                    if (lastj) y[(long)f * S + qp + sJ] = Cvt<TY>::st((typename Cvt<TY>::C)(C(0)));
                 }
             }
         } else {
@@ -271,13 +271,13 @@ __global__ void __launch_bounds__(BJ * 64) k_march(Lvl g, const uint8_t *__restr
             if (!mine) {
                 // copy-through: p, and the domain-boundary faces (no cell of the
                 // colour owns them)
-                y[q] = Cvt<TS>::st(sP[ps][c]);
-                if (i == 0) y[S + q] = Cvt<TS>::st(px[0]);
-                if (i == g.nx - 1) y[S + q + sI] = Cvt<TS>::st(px[3]);
-                if (j == 0) y[2 * S + q] = Cvt<TS>::st(px[1]);
-                if (j == g.ny - 1) y[2 * S + q + sJ] = Cvt<TS>::st(px[4]);
-                if (k == 0) y[3 * S + q] = Cvt<TS>::st(px[2]);
-                if (k == g.nz - 1) y[3 * S + q + 1] = Cvt<TS>::st(px[5]);
+                y[q] = Cvt<TY>::st((typename Cvt<TY>::C)(sP[ps][c]));
+                if (i == 0) y[S + q] = Cvt<TY>::st((typename Cvt<TY>::C)(px[0]));
+                if (i == g.nx - 1) y[S + q + sI] = Cvt<TY>::st((typename Cvt<TY>::C)(px[3]));
+                if (j == 0) y[2 * S + q] = Cvt<TY>::st((typename Cvt<TY>::C)(px[1]));
+                if (j == g.ny - 1) y[2 * S + q + sJ] = Cvt<TY>::st((typename Cvt<TY>::C)(px[4]));
+                if (k == 0) y[3 * S + q] = Cvt<TY>::st((typename Cvt<TY>::C)(px[2]));
+                if (k == g.nz - 1) y[3 * S + q + 1] = Cvt<TY>::st((typename Cvt<TY>::C)(px[5]));
             } else {
                 const C mc = (fb & 1) ? C(1) : C(0);
                 const long qh[3] = {q + sI, q + sJ, q + 1};
@@ -329,12 +329,12 @@ __global__ void __launch_bounds__(BJ * 64) k_march(Lvl g, const uint8_t *__restr
                 }
                 const C dp = (Sc != C(0)) ? quo(tp, Sc) : C(0);
                 const C om = (C)omega;
-                y[q] = Cvt<TS>::st(sP[ps][c] + om * dp);
+                y[q] = Cvt<TY>::st((typename Cvt<TY>::C)(sP[ps][c] + om * dp));
 #pragma unroll
                 for (int d = 0; d < 3; d++) {
                     const C e0 = r0[d] + m0[d] * dp, e1 = r1[d] - m1[d] * dp;
-                    y[(long)(1 + d) * S + q] = Cvt<TS>::st(u0[d] + om * id[d] * (a1[d] * e0 - of[d] * e1));
-                    y[(long)(1 + d) * S + qh[d]] = Cvt<TS>::st(u1[d] + om * id[d] * (-of[d] * e0 + a0[d] * e1));
+                    y[(long)(1 + d) * S + q] = Cvt<TY>::st((typename Cvt<TY>::C)(u0[d] + om * id[d] * (a1[d] * e0 - of[d] * e1)));
+                    y[(long)(1 + d) * S + qh[d]] = Cvt<TY>::st((typename Cvt<TY>::C)(u1[d] + om * id[d] * (-of[d] * e0 + a0[d] * e1)));
                 }
             }
         }
@@ -402,6 +402,15 @@ void march_smooth(const Lvl &g, const uint8_t *fl, const TS *al, const TS *b, co
     dim3 grid = march_grid(g, 4, 2048, ch);
     hipLaunchKernelGGL((k_march<TS, M_SMOOTH>), grid, dim3(256), 0, s, g, fl, al, b, xin, xout, (double *)nullptr,
                        ch, color, omega);
+}
+
+void march_residual_f32(const Lvl &g, const uint8_t *fl, const double *al, const double *b, const double *x, float *r,
+                        double *part, hipStream_t s)
+{
+    int ch;
+    dim3 grid = march_grid(g, 4, 2048, ch);
+    hipLaunchKernelGGL((k_march<double, M_RESID, double, 4, float>), grid, dim3(256), 0, s, g, fl, al, b, x, r, part,
+                       ch, 0, 0.f);
 }
 
 void march_apply_mixed(const Lvl &g, const uint8_t *fl, const double *al, const float *x, double *y, hipStream_t s)

@@ -455,7 +455,10 @@ template <typename TV, typename TW> static int fgmres(st_ctx *c, const double *b
     const int rb = (int)residual_blocks(g);
     for (;;) {
         double *r0 = std::is_same<TV, double>::value ? c->V : c->Wv;  // FP64 residual
-        launch_residual<double>(g, c->fl[0], c->h64.al[0], b, x, r0, c->part, c->s);
+        // FP32 path on a tiled level: the FP64 residual goes straight into u_0 in FP32
+        const bool r_f32 = LAZY && std::is_same<TV, float>::value && march_ok(g);
+        if (r_f32) march_residual_f32(g, c->fl[0], c->h64.al[0], b, x, reinterpret_cast<float *>(Vb), c->part, c->s);
+        else launch_residual<double>(g, c->fl[0], c->h64.al[0], b, x, r0, c->part, c->s);
         launch_sum(rb, 1, c->part, c->red, c->s);
         double r2 = 0.0;
         CK(hipMemcpyAsync(&r2, c->red, sizeof(double), hipMemcpyDeviceToHost, c->s));
@@ -469,7 +472,7 @@ template <typename TV, typename TW> static int fgmres(st_ctx *c, const double *b
             sig[0] = 1.0 / beta;
             sig2[0] = sig[0] * sig[0];
             CK(hipMemcpyAsync(c->sig2_dev, sig2.data(), sizeof(double), hipMemcpyHostToDevice, c->s));
-            if (!std::is_same<TV, double>::value) launch_axpby<double, TV>(nv, 1.0, r0, 0.0, Vb, c->s);
+            if (!std::is_same<TV, double>::value && !r_f32) launch_axpby<double, TV>(nv, 1.0, r0, 0.0, Vb, c->s);
         } else {
             launch_axpby<double, TV>(nv, 1.0 / beta, r0, 0.0, Vb, c->s);
         }
