@@ -1,7 +1,8 @@
 """CPU FP64 oracle (ctypes wrapper over oracle/liborc.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py, as the checker and the CPU baseline.  The product
+cpu_baseline leg of bench.py, as the reference results tests compare against and
+as the CPU baseline.  The product
 package (stokes_to_amd) never imports it.
 
 Arrays use the ABI layout: C-order (i, j, k), k fastest; a dof vector is

@@ -2,7 +2,8 @@
  * CPU FP64 oracle for the Stokes/Brinkman topology-optimization hot path.
  *
  * TEST INFRASTRUCTURE ONLY: used by tests/, __graft_entry__.smoke() and the
- * cpu_baseline leg of bench.py as the checker and as the CPU baseline.  The
+ * cpu_baseline leg of bench.py, as the results tests compare against and as the
+ * CPU baseline.  The
  * product path (stokes_to_amd/, include/) never links or calls it.
  *
  * Every operator here is written as the reference composes it: separate DEC

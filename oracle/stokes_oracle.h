@@ -3,7 +3,8 @@
  *
  * TEST INFRASTRUCTURE ONLY.  Nothing in the product (stokes_to_amd/, include/)
  * links, loads or calls this code; only tests/, __graft_entry__.smoke() and the
- * cpu_baseline leg of bench.py use it, and only as the checker / CPU baseline.
+ * cpu_baseline leg of bench.py use it, and only as the results tests compare
+ * against / the CPU baseline.
  *
  * Layout (the ABI layout, identical to the reference's host Field/FaceField,
  * SX/reservoir/Field.h:67-72): every array is C-order over (i, j, k) with k

@@ -978,7 +978,7 @@ static void remember_design(st_ctx *c)
 //   without one:
 //     x_D + P P net(rho)                   against  x_D (the zero initial guess).
 // The candidate with the smaller true FP64 residual ||b - A x|| is kept, so the
-// plain method remains the judge of every prediction.  After 3 rejections in a row
+// start without the network wins every tie.  After 3 rejections in a row
 // the candidate is built and checked only on every 4th call (the others take the
 // start without the network), so a predictor that keeps losing costs little.
 static int learned_guess(st_ctx *c, float *ms, int *used)
