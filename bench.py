@@ -168,6 +168,17 @@ def run_to(args, prob):
                   f"used {sl['learned_used']}")
     torch.cuda.synchronize()
     region_ms = (time.perf_counter() - t_region) * 1e3 if t_region is not None else 0.0
+    # untimed: the last design solved from a cold start, without and with the network
+    # (no previous solution held: the learned start is x_D + P P net(rho))
+    cold = {}
+    for mode in ("zero", "learned"):
+        L.reset_solution()
+        _, sc = L.solve(init=mode, fetch=False)
+        cold[mode] = {"iters": int(sc["iters"]), "ms": round(sc["ms"], 3), "relres0": sc["relres0"],
+                      "learned_used": int(sc["learned_used"])}
+    log(args, f"[bench] cold start on the last design: zero {cold['zero']['iters']} it "
+              f"(r0 {cold['zero']['relres0']:.2e}), learned {cold['learned']['iters']} it "
+              f"(r0 {cold['learned']['relres0']:.2e}, taken {cold['learned']['learned_used']})")
     Z.lib.st_profile(Z.h, 0)
     K.lib.st_profile(K.h, 0)
     prof = read_profile(Z)
@@ -176,7 +187,7 @@ def run_to(args, prob):
     extra = {"train_ms_per_iter": train_ms, "train_ms_total": float(sum(train_ms)),
              "inference_ms_median": float(np.median(inf_ms)) if inf_ms else 0.0,
              "learned_taken": int(sum(used)), "prof": prof, "prof_mp": prof_mp, "last_design": last_rhof,
-             "loss_vs_zero_median": float(np.median(lossr)) if lossr else None,
+             "loss_vs_zero_median": float(np.median(lossr)) if lossr else None, "cold_start": cold,
              "final_J": r["J"], "final_vol": r["vol"], "region_ms": region_ms}
     for s in (L, Z, K):
         s.close()
@@ -324,7 +335,7 @@ def main():
             "speedup_vs_plain": round(plain["median_ms"] / value, 3),
             "speedup_keep_vs_plain": round(plain["median_ms"] / keep["median_ms"], 3),
             "learned": {k: extra[k] for k in ("train_ms_total", "train_ms_per_iter", "inference_ms_median",
-                                              "learned_taken", "loss_vs_zero_median")},
+                                              "learned_taken", "loss_vs_zero_median", "cold_start")},
             "final_objective": extra["final_J"], "final_volume": extra["final_vol"],
             "timed_region_wall_ms": round(extra["region_ms"], 1),
             "timed_region_note": "wall time of the timed iterations, each solving the design three times "
