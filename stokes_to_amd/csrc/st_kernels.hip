@@ -207,16 +207,10 @@ __global__ void __launch_bounds__(256) k_residual(Lvl g, const uint8_t *fl, cons
 // form through the scalar Schur complement.  xin -> xout (ping-pong); DOFs not
 // owned by a cell of the colour are copied.  ZERO: xin is identically zero.
 template <typename TS, int DIM, bool ZERO>
-__device__ __forceinline__ void smooth_body(const Lvl &g, const uint8_t *fl, const TS *al, const TS *b,
-                                            const TS *xin, TS *xout, int color, float omega, long t)
+__device__ __forceinline__ void smooth_cell(const Lvl &g, const uint8_t *fl, const TS *al, const TS *b,
+                                            const TS *xin, TS *xout, int color, float omega, int i, int j, int k)
 {
     typedef typename Cvt<TS>::C C;
-    long ncell = (long)g.nx * g.ny * g.nz;
-    if (t >= ncell) return;
-    int k = (int)(t % g.nz);
-    long tt = t / g.nz;
-    int j = (int)(tt % g.ny);
-    int i = (int)(tt / g.ny);
     Sten<TS, DIM> s{g, fl, xin, al, -1};
     const long q = s.id(i, j, k);
     const long S = g.S;
@@ -289,6 +283,37 @@ __device__ __forceinline__ void smooth_body(const Lvl &g, const uint8_t *fl, con
         xout[(long)(1 + d) * S + q] = Cvt<TS>::st(u0 + om * id * (a1[d] * e0 - of[d] * e1));
         xout[(long)(1 + d) * S + qh] = Cvt<TS>::st(u1 + om * id * (-of[d] * e0 + a0[d] * e1));
     }
+}
+
+// one thread per cell t (k fastest)
+template <typename TS, int DIM, bool ZERO>
+__device__ __forceinline__ void smooth_body(const Lvl &g, const uint8_t *fl, const TS *al, const TS *b,
+                                            const TS *xin, TS *xout, int color, float omega, long t)
+{
+    const long ncell = (long)g.nx * g.ny * g.nz;
+    if (t >= ncell) return;
+    const int k = (int)(t % g.nz);
+    const long tt = t / g.nz;
+    smooth_cell<TS, DIM, ZERO>(g, fl, al, b, xin, xout, color, omega, (int)(tt / g.ny), (int)(tt % g.ny), k);
+}
+
+// The pass from x = 0 in 3D with one thread per pair of cells (k, k+1), k even: the
+// pair holds one cell of each colour, so every lane runs the block solve of its
+// colour cell (one thread per cell leaves half the lanes idle through it)
+template <typename TS>
+__global__ void __launch_bounds__(256) k_smooth0_pairs(Lvl g, const uint8_t *fl, const TS *al, const TS *b,
+                                                       TS *xout, int color, float omega)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int hz = g.nz / 2;
+    if (t >= (long)g.nx * g.ny * hz) return;
+    const int kp = (int)(t % hz);
+    const long tt = t / hz;
+    const int j = (int)(tt % g.ny), i = (int)(tt / g.ny);
+    const int k0 = 2 * kp;
+    const int km = (((i + j + k0) & 1) == color) ? k0 : k0 + 1;
+    smooth_cell<TS, 3, true>(g, fl, al, b, (const TS *)nullptr, xout, color, omega, i, j, km);
+    smooth_cell<TS, 3, true>(g, fl, al, b, (const TS *)nullptr, xout, color, omega, i, j, 2 * k0 + 1 - km);
 }
 
 template <typename TS, int DIM, bool ZERO>
@@ -1129,6 +1154,10 @@ void launch_smooth(const Lvl &g, const uint8_t *fl, const TS *al, const TS *b, c
 {
     long nc = (long)g.nx * g.ny * g.nz;
     if (!zero && march_ok(g)) { march_smooth<TS>(g, fl, al, b, xin, xout, color, omega, s); return; }
+    if (zero && g.dim == 3 && g.nz % 2 == 0) {
+        hipLaunchKernelGGL((k_smooth0_pairs<TS>), dim3(nb(nc / 2)), dim3(256), 0, s, g, fl, al, b, xout, color, omega);
+        return;
+    }
     if (g.dim == 3) {
         if (zero) hipLaunchKernelGGL((k_smooth<TS, 3, true>), dim3(nb(nc)), dim3(256), 0, s, g, fl, al, b, xin, xout, color, omega);
         else hipLaunchKernelGGL((k_smooth<TS, 3, false>), dim3(nb(nc)), dim3(256), 0, s, g, fl, al, b, xin, xout, color, omega);
