@@ -209,3 +209,24 @@ def test_fine_level_solve_64(make_solver, orc, dim, n, prec):
     b = orc.lift_rhs(dim, P.shape, fixed, P.dirichlet)
     r = b - orc.apply(dim, P.shape, fixed, vol, pen, x)
     assert np.linalg.norm(r) / np.linalg.norm(b) < 1.5e-8
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_solution_is_linear_in_the_boundary_data(make_solver, prec):
+    """Dirichlet data scaled by 1e14 must give 1e14 times the solution.  The Krylov basis
+    is stored unnormalised, so its vectors reach norms far beyond 1e12 here: the rescale
+    that keeps them in range (and, in FP32, inside the float range) must not change the
+    result.  64^3 runs the tiled kernels, including the FP32 restart residual."""
+    n = 64
+    P, rho = setup(3, n)
+    a = solver_for(make_solver, P, rho, rtol=1e-10, precision=prec)
+    x1, st1 = a.solve()
+    b = make_solver(3, n, rtol=1e-10, precision=prec)
+    b.init_boundary(P.fixed)
+    b.update_b(dirichlet=P.dirichlet * 1e14)
+    b.set_design(rho)
+    x2, st2 = b.solve()
+    assert st1["converged"] and st2["converged"]
+    assert rel(x2, 1e14 * x1) < 1e-8
+    assert abs(st2["iters"] - st1["iters"]) <= 1
+
