@@ -230,3 +230,16 @@ def test_solution_is_linear_in_the_boundary_data(make_solver, prec):
     assert rel(x2, 1e14 * x1) < 1e-8
     assert abs(st2["iters"] - st1["iters"]) <= 1
 
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "bf16"])
+def test_solve_is_bitwise_reproducible(make_solver, prec):
+    """Every reduction (inner products, residual norms) sums fixed partials in a fixed
+    order and no kernel accumulates with atomics, so the same solve gives the same bits."""
+    P, rho = setup(3, 64)
+    s = solver_for(make_solver, P, rho, rtol=1e-8, precision=prec)
+    x1, st1 = s.solve()
+    s.reset_solution()
+    x2, st2 = s.solve()
+    assert st1["iters"] == st2["iters"]
+    assert np.array_equal(x1, x2)
+
