@@ -56,7 +56,8 @@ typedef struct {
     int restart;         /* FGMRES restart length (<= 32) */
     int nu_pre, nu_post; /* smoothing sweeps (each = both colours) */
     double omega;        /* smoother damping */
-    int closed;          /* add the all-ones pressure gauge on the coarsest level */
+    int closed;          /* all-ones pressure gauge on the coarsest level (initial value;
+                            st_set_geometry's closed argument sets it) */
     double filter_radius;/* density filter radius in cells (<= 1 disables) */
     int optimizer;       /* ST_OPT_OC (default) or ST_OPT_MMA, the reference's optimizer */
 } st_params;
@@ -86,9 +87,13 @@ int st_get_params(const st_ctx *c, st_params *p);
 long st_num_dofs(const st_ctx *c);
 long st_num_cells(const st_ctx *c);
 
-/* replaces FluidSteadyState3D::init_boundary (ST/FluidSteadyState3D.cpp:92-114):
-   fixed[cells|faces], 1 = fixed (Dirichlet) DOF; vol is the free mask */
-int st_set_geometry(st_ctx *c, const uint8_t *fixed);
+/* replaces FluidSteadyState3D::init_boundary(cell_vol, cell_fixed, face_vol, face_fixed,
+   bool _closed) (ST/FluidSteadyState3D.h:33-34, ST/FluidSteadyState3D.cpp:92-114):
+   fixed[cells|faces], 1 = fixed (Dirichlet) DOF; vol is the free mask.  closed = 1 adds
+   the all-ones pressure gauge to the coarsest matrix (a domain whose boundary is fixed
+   everywhere, ST/StokeLike.cu:605-608), 0 removes it, -1 keeps params.closed.  The
+   coarsest inverse is rebuilt on the next solve. */
+int st_set_geometry(st_ctx *c, const uint8_t *fixed, int closed);
 /* replaces FluidSteadyState3D::update_penalty (ST/FluidSteadyState3D.cpp:130-150):
    pen[cells|faces]; cell penalties must be zero */
 int st_set_penalty(st_ctx *c, const double *pen);
@@ -165,7 +170,9 @@ int st_to_step(st_ctx *c, int init, double volfrac, double move, double eta, dou
                double *vol, double *change, st_stats *fwd, st_stats *adj);
 
 /* ---- kernel timing (HIP events around each launch on the context stream) ---- */
-enum { ST_PROF_SMOOTH = 0, ST_PROF_APPLY = 1, ST_PROF_RESIDUAL = 2, ST_PROF_NKIND = 3 };
+enum { ST_PROF_SMOOTH = 0, ST_PROF_APPLY = 1, ST_PROF_RESIDUAL = 2, ST_PROF_SMOOTH0 = 3, ST_PROF_NKIND = 4 };
+/* ST_PROF_SMOOTH: smoother colour passes with x given (k_march<., SMOOTH>);
+   ST_PROF_SMOOTH0: the first pass of a V-cycle, from x = 0 (k_smooth0_pairs) */
 /* on != 0 starts recording the finest-level launches of each kind; 0 stops */
 int st_profile(st_ctx *c, int on);
 /* total device ms and launch count recorded for `kind` since st_profile(c, 1);

@@ -237,6 +237,12 @@ void orc_apply(int dim, int nx, int ny, int nz, const uint8_t *fixed, const doub
 void orc_energy_op(int dim, int nx, int ny, int nz, const uint8_t *fixed, const double *vol,
                    const double *pen, const double *x, double *y)
 {
+    /* the reference's 2D objective is another operator than its 3D one: the face
+     * 5-point Laplacian of FT/FluidEnergyGrad.cu:30-61 (Appendix B 5) */
+    if (dim == 2) {
+        orc_energy_op_ref2d(nx, ny, fixed, pen, x, y);
+        return;
+    }
     G g = mkg(dim, nx, ny, nz);
     memset(y, 0, sizeof(double) * g.n);
     velocity_terms(&g, fixed, vol, x, y);
@@ -679,8 +685,12 @@ void orc_restrict(int dim, int nx, int ny, int nz, const uint8_t *cfixed, int ta
 }
 
 /* one colour step of the box-Vanka smoother: for every cell of the colour, solve the
- * local 7x7 (5x5 in 2D) block [K_d, b_f; b_c^T, s0] exactly (the velocity part is
- * block diagonal per axis for this operator) and add omega*correction. */
+ * local 7x7 (5x5 in 2D) block [K, b_f; mc b_f^T, 1 - mc] exactly and add
+ * omega*correction.  With E = blockdiag_d [[a_lo + (1-mc) m_lo, -m_lo m_hi],
+ * [-m_lo m_hi, a_hi + (1-mc) m_hi]] and b_f = (-m_lo, m_hi) per axis, the velocity block
+ * is K = E - (1 - mc) b_f b_f^T: in a free cell (mc = 1) the cross-axis couplings of
+ * curl-curl and grad-div cancel and K = E; in a fixed cell the curl-curl couplings
+ * remain (rank one), and the block is solved by Sherman-Morrison. */
 void orc_smooth_color(int dim, int nx, int ny, int nz, const uint8_t *fixed, const double *pen,
                       const double *b, const double *xin, double *xout, int color, double omega)
 {
@@ -697,8 +707,8 @@ void orc_smooth_color(int dim, int nx, int ny, int nz, const uint8_t *fixed, con
         long c = ((long)i * g.ny + j) * g.nz + k;
         double mc = fixed[c] ? 0.0 : 1.0, s0 = pen[c] + (1.0 - mc);
         int ci[3] = {i, j, k};
-        long fq[3][2]; double K[3][3], bfv[3][2], bcv[3][2], rd[3][2];
-        double S = s0, t = r[c];
+        long fq[3][2]; double K[3][3], bfv[3][2], rd[3][2];
+        double beta = 0.0, tb = 0.0; /* b^T E^-1 b, b^T E^-1 r_u */
         for (int d = 0; d < dim; d++) {
             int A, B, C; fdims(&g, d, &A, &B, &C);
             double m[2], dg[2];
@@ -714,28 +724,34 @@ void orc_smooth_color(int dim, int nx, int ny, int nz, const uint8_t *fixed, con
                     if (cc[0] < 0 || cc[1] < 0 || cc[2] < 0 || cc[0] >= g.nx || cc[1] >= g.ny || cc[2] >= g.nz) continue;
                     nadj += fixed[((long)cc[0] * g.ny + cc[1]) * g.nz + cc[2]] ? 0.0 : 1.0;
                 }
-                dg[s] = m[s] * (nadj + ne) + pen[q] + (1.0 - m[s]);
+                /* diagonal of A: free-cell count + edge count + penalty (identity on a
+                 * fixed face), plus the (1-mc) m term that b b^T takes back in K */
+                dg[s] = m[s] * (nadj + ne) + pen[q] + (1.0 - m[s]) + (1.0 - mc) * m[s];
                 rd[d][s] = r[q];
             }
-            double off = -m[0] * mc * m[1];
+            double off = -m[0] * m[1];
             K[d][0] = dg[0]; K[d][1] = off; K[d][2] = dg[1];
             bfv[d][0] = -m[0]; bfv[d][1] = m[1];
-            bcv[d][0] = -mc * m[0]; bcv[d][1] = mc * m[1];
             double det = dg[0] * dg[1] - off * off, id = 1.0 / det;
-            /* K^-1 = id*[[dg1,-off],[-off,dg0]] */
+            /* E_d^-1 = id*[[dg1,-off],[-off,dg0]] */
             double kb0 = id * (dg[1] * bfv[d][0] - off * bfv[d][1]);
             double kb1 = id * (-off * bfv[d][0] + dg[0] * bfv[d][1]);
             double kr0 = id * (dg[1] * rd[d][0] - off * rd[d][1]);
             double kr1 = id * (-off * rd[d][0] + dg[0] * rd[d][1]);
-            S -= bcv[d][0] * kb0 + bcv[d][1] * kb1;
-            t -= bcv[d][0] * kr0 + bcv[d][1] * kr1;
+            beta += bfv[d][0] * kb0 + bfv[d][1] * kb1;
+            tb += bfv[d][0] * kr0 + bfv[d][1] * kr1;
         }
-        double dp = (fabs(S) > 1e-300) ? t / S : 0.0;
+        /* free cell: Schur complement -beta (+ the cell penalty s0 = pen_c);
+         * fixed cell: the identity row gives dp = r_p, and the velocity solve of
+         * K = E - b b^T by Sherman-Morrison is E^-1 (r_u - b g) with g = (dp - tb)/(1 - beta) */
+        double S = mc * (s0 - beta) + (1.0 - mc) * s0;
+        double dp = (fabs(S) > 1e-300) ? (r[c] - mc * tb) / S : 0.0;
+        double gm = (mc != 0.0) ? dp : (dp - tb) / (1.0 - beta);
         xout[c] += omega * dp;
         for (int d = 0; d < dim; d++) {
             double dg0 = K[d][0], off = K[d][1], dg1 = K[d][2];
             double id = 1.0 / (dg0 * dg1 - off * off);
-            double r0 = rd[d][0] - bfv[d][0] * dp, r1 = rd[d][1] - bfv[d][1] * dp;
+            double r0 = rd[d][0] - bfv[d][0] * gm, r1 = rd[d][1] - bfv[d][1] * gm;
             xout[fq[d][0]] += omega * id * (dg1 * r0 - off * r1);
             xout[fq[d][1]] += omega * id * (-off * r0 + dg0 * r1);
         }

@@ -42,10 +42,14 @@ void orc_apply(int dim, int nx, int ny, int nz, const uint8_t *fixed, const doub
  * unmasked curl input, no definite fix */
 void orc_apply_ref2d(int nx, int ny, const uint8_t *fixed, const double *vol,
                      const double *pen, const double *x, double *y);
-/* objective operator G (FT/FluidEnergyGrad3D.cu:27-134, 3D form for dim 2/3) */
+/* objective operator G: dim 3 FT/FluidEnergyGrad3D.cu:27-134 (grad-div + curl-curl on the
+ * masked velocity, plus the penalty); dim 2 the reference's 2D form, orc_energy_op_ref2d */
 void orc_energy_op(int dim, int nx, int ny, int nz, const uint8_t *fixed, const double *vol,
                    const double *pen, const double *x, double *y);
-/* literal 2D reference objective operator (FT/FluidEnergyGrad.cu:30-61) */
+/* literal 2D reference objective operator (FT/FluidEnergyGrad.cu:30-61): face 5-point
+ * Laplacian 4u - sum of the 4 same-component neighbours (zero outside the face array,
+ * CX/form01mapping.cu:429-525) of the unmasked velocity, fixed rows zeroed, lap_coeff 1,
+ * plus pen * x on every dof */
 void orc_energy_op_ref2d(int nx, int ny, const uint8_t *fixed, const double *pen,
                          const double *x, double *y);
 /* right-hand side that lifts Dirichlet values given on fixed faces */

@@ -57,7 +57,7 @@ SIGNATURES = {
     "st_get_params": (I, [C, ctypes.POINTER(StParams)]),
     "st_num_dofs": (ctypes.c_long, [C]),
     "st_num_cells": (ctypes.c_long, [C]),
-    "st_set_geometry": (I, [C, U8]),
+    "st_set_geometry": (I, [C, U8, I]),
     "st_set_penalty": (I, [C, D]),
     "st_set_design": (I, [C, D]),
     "st_get_design": (I, [C, D]),

@@ -65,14 +65,23 @@ def build(jobs=4, verbose=False):
     return LIB
 
 
+BINDING_TEST = os.path.join(HERE, "binding_test0")
+
+
+def _host_program(src, out):
+    """Host C++ on the C ABI only (g++), linked against the library next to it."""
+    tmp = out + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-Wall", f"-I{os.path.join(ROOT, 'include')}", src, "-o", tmp, f"-L{HERE}",
+          "-l:libstokes_hip.so", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, out)
+    return out
+
+
 def build_driver():
-    """The fluid_topo command-line driver: host C++ on the C ABI only (g++), linked
-    against the library next to it."""
-    src = os.path.join(HERE, "driver", "fluid_topo.cpp")
-    tmp = DRIVER + ".tmp"
-    _run(["g++", "-O2", "-std=c++17", "-Wall", src, "-o", tmp, f"-L{HERE}", "-l:libstokes_hip.so",
-          "-Wl,-rpath,$ORIGIN"])
-    os.replace(tmp, DRIVER)
+    """The fluid_topo command-line driver and the program that runs the reference's
+    FluidSteadyState3D call sequence through the C++ binding (hip_steady_state3d.h)."""
+    _host_program(os.path.join(HERE, "driver", "fluid_topo.cpp"), DRIVER)
+    _host_program(os.path.join(HERE, "driver", "binding_test0.cpp"), BINDING_TEST)
     return DRIVER
 
 

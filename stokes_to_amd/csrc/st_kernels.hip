@@ -109,6 +109,24 @@ struct Sten {
         C u = ldx((long)(1 + d) * g.S + q);
         return visc<d>(i, j, k) + Cvt<TS>::ld(al[(long)d * g.S + q]) * u;
     }
+    // 2D objective operator of FT/FluidEnergyGrad.cu:30-61: the face 5-point Laplacian
+    // 4u - (4 neighbours of the same component) of the *unmasked* velocity, zero outside
+    // the face array (grid2DOperator::faceLapMapping, CX/form01mapping.cu:429-525),
+    // rows of fixed faces zeroed, times lap_coeff = 1, plus the penalty alpha_f u_f
+    template <int d>
+    __device__ __forceinline__ C Gface2(int i, int j) const
+    {
+        long q = id(i, j, 0);
+        if (!((fl[q] >> (1 + d)) & 1)) return C(0);
+        const int A = g.nx + (d == 0), B = g.ny + (d == 1);  // face array dims
+        auto u = [&](int a, int b) -> C {
+            return ((unsigned)a < (unsigned)A && (unsigned)b < (unsigned)B) ? ldx((long)(1 + d) * g.S + id(a, b, 0))
+                                                                            : C(0);
+        };
+        const C uc = u(i, j);
+        const C lap = C(4) * uc - u(i - 1, j) - u(i + 1, j) - u(i, j - 1) - u(i, j + 1);
+        return lap + Cvt<TS>::ld(al[(long)d * g.S + q]) * uc;
+    }
 };
 
 __device__ __forceinline__ void node_ijk(const Lvl &g, long n, int &i, int &j, int &k)
@@ -200,11 +218,15 @@ __global__ void __launch_bounds__(256) k_residual(Lvl g, const uint8_t *fl, cons
 
 // One colour of the red-black box-Vanka smoother (the reference smooths the same
 // cell blocks [6 faces + cell] with batched LU, ST/StokeLike.cu:41-372,
-// CX/DescentSmoother.cu:31-53).  For a cell c of the colour the local block is
-//   [K_d  b_f,d ; b_c,d^T  s0],  K_d = [[a_lo, -m_lo m_c m_hi], [., a_hi]]
-// which is exact for this operator (cross-axis couplings inside the cell cancel
-// between grad-div and curl-curl when vol = free mask).  It is solved in closed
-// form through the scalar Schur complement.  xin -> xout (ping-pong); DOFs not
+// CX/DescentSmoother.cu:31-53).  For a cell c of the colour the local block of A is
+//   [K  b_f ; m_c b_f^T  1 - m_c],  K = E - (1 - m_c) b_f b_f^T,
+//   E = blockdiag_d [[a_lo + (1-m_c) m_lo, -m_lo m_hi], [-m_lo m_hi, a_hi + (1-m_c) m_hi]],
+//   b_f = (-m_lo, m_hi) per axis.
+// In a free cell the cross-axis couplings of curl-curl and grad-div cancel (vol = free
+// mask) and K = E; the block is solved through the scalar Schur complement.  In a
+// fixed cell the curl-curl couplings remain as the rank-one term, p is decoupled
+// (dp = r_p) and the velocity solve is Sherman-Morrison on E.  Both are the exact
+// block inverse (equal to LU of the probed block).  xin -> xout (ping-pong); DOFs not
 // owned by a cell of the colour are copied.  ZERO: xin is identically zero.
 template <typename TS, int DIM, bool ZERO>
 __device__ __forceinline__ void smooth_cell(const Lvl &g, const uint8_t *fl, const TS *al, const TS *b,
@@ -232,8 +254,7 @@ __device__ __forceinline__ void smooth_cell(const Lvl &g, const uint8_t *fl, con
     const C NE = (DIM == 3) ? C(4) : C(2);
     C rp = Cvt<TS>::ld(b[q]);
     if (!ZERO) rp -= s.Acell(i, j, k);
-    C Sc = C(1) - mc;
-    C tp = rp;
+    C beta = C(0), tb = C(0);  // b^T E^-1 b, b^T E^-1 r_u
     C a0[3], a1[3], of[3], r0[3], r1[3], m0[3], m1[3];
 #pragma unroll
     for (int d = 0; d < DIM; d++) {
@@ -247,9 +268,9 @@ __device__ __forceinline__ void smooth_cell(const Lvl &g, const uint8_t *fl, con
         C c_hi = (C)(s.F(i + di, j + dj, k + dk) & 1);
         C al0 = Cvt<TS>::ld(al[(long)d * S + q]);
         C al1 = Cvt<TS>::ld(al[(long)d * S + qh]);
-        a0[d] = m0[d] * (c_lo + mc + NE) + al0 + (C(1) - m0[d]);
-        a1[d] = m1[d] * (mc + c_hi + NE) + al1 + (C(1) - m1[d]);
-        of[d] = -m0[d] * mc * m1[d];
+        a0[d] = m0[d] * (c_lo + mc + NE) + al0 + (C(1) - m0[d]) + (C(1) - mc) * m0[d];
+        a1[d] = m1[d] * (mc + c_hi + NE) + al1 + (C(1) - m1[d]) + (C(1) - mc) * m1[d];
+        of[d] = -m0[d] * m1[d];
         C rl = Cvt<TS>::ld(b[(long)(1 + d) * S + q]);
         C rh = Cvt<TS>::ld(b[(long)(1 + d) * S + qh]);
         if (!ZERO) {
@@ -265,10 +286,12 @@ __device__ __forceinline__ void smooth_cell(const Lvl &g, const uint8_t *fl, con
         const C kb1 = id * (-of[d] * (-m0[d]) + a0[d] * m1[d]);
         const C kr0 = id * (a1[d] * rl - of[d] * rh);
         const C kr1 = id * (-of[d] * rl + a0[d] * rh);
-        Sc -= mc * ((-m0[d]) * kb0 + m1[d] * kb1);
-        tp -= mc * ((-m0[d]) * kr0 + m1[d] * kr1);
+        beta += (-m0[d]) * kb0 + m1[d] * kb1;
+        tb += (-m0[d]) * kr0 + m1[d] * kr1;
     }
-    const C dp = (Sc != C(0)) ? tp / Sc : C(0);
+    const C Sc = (mc != C(0)) ? -beta : C(1);
+    const C dp = (Sc != C(0)) ? (rp - mc * tb) / Sc : C(0);
+    const C gm = (mc != C(0)) ? dp : (dp - tb) / (C(1) - beta);
     const C om = (C)omega;
     const C p0 = ZERO ? C(0) : Cvt<TS>::ld(xin[q]);
     xout[q] = Cvt<TS>::st(p0 + om * dp);
@@ -276,8 +299,8 @@ __device__ __forceinline__ void smooth_cell(const Lvl &g, const uint8_t *fl, con
     for (int d = 0; d < DIM; d++) {
         const long qh = q + st[d];
         const C id = C(1) / (a0[d] * a1[d] - of[d] * of[d]);
-        const C e0 = r0[d] + m0[d] * dp;   // r - b_f dp, b_f = (-m0, m1)
-        const C e1 = r1[d] - m1[d] * dp;
+        const C e0 = r0[d] + m0[d] * gm;   // r - b_f g, b_f = (-m0, m1)
+        const C e1 = r1[d] - m1[d] * gm;
         const C u0 = ZERO ? C(0) : Cvt<TS>::ld(xin[(long)(1 + d) * S + q]);
         const C u1 = ZERO ? C(0) : Cvt<TS>::ld(xin[(long)(1 + d) * S + qh]);
         xout[(long)(1 + d) * S + q] = Cvt<TS>::st(u0 + om * id * (a1[d] * e0 - of[d] * e1));
@@ -910,7 +933,8 @@ __global__ void __launch_bounds__(256) k_lift(Lvl g, const uint8_t *fl, const do
     if (DIM == 3) b[3 * g.S + n] = ((fb >> 3) & 1) ? body[3 * g.S + n] - s.template visc<2>(i, j, k) : xd[3 * g.S + n];
 }
 
-// y = G x (optional) and partial sums of x.Gx
+// y = G x (optional) and partial sums of x.Gx: 3D FT/FluidEnergyGrad3D.cu:27-134,
+// 2D FT/FluidEnergyGrad.cu:30-61 (J = x.Gx / 2, FT/FluidEnergy{,3D}.cpp:95-109)
 template <int DIM>
 __global__ void __launch_bounds__(256) k_energy(Lvl g, const uint8_t *fl, const double *al, const double *x, double *y,
                                                 double *part)
@@ -922,8 +946,15 @@ __global__ void __launch_bounds__(256) k_energy(Lvl g, const uint8_t *fl, const 
         int i, j, k;
         node_ijk(g, n, i, j, k);
         Sten<double, DIM> s{g, fl, x, al, -1};
-        double v1 = s.template Gface<0>(i, j, k), v2 = s.template Gface<1>(i, j, k), v3 = 0.0;
-        if (DIM == 3) v3 = s.template Gface<2>(i, j, k);
+        double v1, v2, v3 = 0.0;
+        if (DIM == 3) {
+            v1 = s.template Gface<0>(i, j, k);
+            v2 = s.template Gface<1>(i, j, k);
+            v3 = s.template Gface<2>(i, j, k);
+        } else {  // the reference's 2D objective differs from its 3D one (face Laplacian)
+            v1 = s.template Gface2<0>(i, j);
+            v2 = s.template Gface2<1>(i, j);
+        }
         acc = x[g.S + n] * v1 + x[2 * g.S + n] * v2 + (DIM == 3 ? x[3 * g.S + n] * v3 : 0.0);
         if (y) {
             y[n] = 0.0;

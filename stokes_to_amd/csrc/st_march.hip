@@ -311,28 +311,32 @@ __global__ void __launch_bounds__(BJ * 64) k_march(Lvl g, const uint8_t *__restr
                     const int slot1 = (d == 0) ? pn : ps;
                     u1[d] = (m1[d] != C(0)) ? sW[slot1][d][cw1[d]] : px[3 + d];
                     const C al0 = pa[d], al1 = pa[3 + d];
-                    a0[d] = m0[d] * (clo[d] + mc + C(4)) + al0 + (C(1) - m0[d]);
-                    a1[d] = m1[d] * (mc + chi[d] + C(4)) + al1 + (C(1) - m1[d]);
+                    a0[d] = m0[d] * (clo[d] + mc + C(4)) + al0 + (C(1) - m0[d]) + (C(1) - mc) * m0[d];
+                    a1[d] = m1[d] * (mc + chi[d] + C(4)) + al1 + (C(1) - m1[d]) + (C(1) - mc) * m1[d];
                 }
-                C Sc = C(1) - mc, tp = rp;
+                // block solve of st_kernels.hip smooth_cell: Schur complement in a free
+                // cell, Sherman-Morrison (K = E - b b^T) in a fixed cell
+                C beta = C(0), tb = C(0);
                 C of[3], id[3];
 #pragma unroll
                 for (int d = 0; d < 3; d++) {
-                    of[d] = -m0[d] * mc * m1[d];
+                    of[d] = -m0[d] * m1[d];
                     id[d] = rcp(a0[d] * a1[d] - of[d] * of[d]);
                     const C kb0 = id[d] * (a1[d] * (-m0[d]) - of[d] * m1[d]);
                     const C kb1 = id[d] * (-of[d] * (-m0[d]) + a0[d] * m1[d]);
                     const C kr0 = id[d] * (a1[d] * r0[d] - of[d] * r1[d]);
                     const C kr1 = id[d] * (-of[d] * r0[d] + a0[d] * r1[d]);
-                    Sc -= mc * ((-m0[d]) * kb0 + m1[d] * kb1);
-                    tp -= mc * ((-m0[d]) * kr0 + m1[d] * kr1);
+                    beta += (-m0[d]) * kb0 + m1[d] * kb1;
+                    tb += (-m0[d]) * kr0 + m1[d] * kr1;
                 }
-                const C dp = (Sc != C(0)) ? quo(tp, Sc) : C(0);
+                const C Sc = (mc != C(0)) ? -beta : C(1);
+                const C dp = (Sc != C(0)) ? quo(rp - mc * tb, Sc) : C(0);
+                const C gm = (mc != C(0)) ? dp : quo(dp - tb, C(1) - beta);
                 const C om = (C)omega;
                 y[q] = Cvt<TY>::st((typename Cvt<TY>::C)(sP[ps][c] + om * dp));
 #pragma unroll
                 for (int d = 0; d < 3; d++) {
-                    const C e0 = r0[d] + m0[d] * dp, e1 = r1[d] - m1[d] * dp;
+                    const C e0 = r0[d] + m0[d] * gm, e1 = r1[d] - m1[d] * gm;
                     y[(long)(1 + d) * S + q] = Cvt<TY>::st((typename Cvt<TY>::C)(u0[d] + om * id[d] * (a1[d] * e0 - of[d] * e1)));
                     y[(long)(1 + d) * S + qh[d]] = Cvt<TY>::st((typename Cvt<TY>::C)(u1[d] + om * id[d] * (-of[d] * e0 + a0[d] * e1)));
                 }

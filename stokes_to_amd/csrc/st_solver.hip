@@ -80,9 +80,9 @@ struct st_ctx {
     // kernel timing: event pairs around finest-level launches
     bool prof = false;
     std::vector<hipEvent_t> evpool[ST_PROF_NKIND];
-    size_t evused[ST_PROF_NKIND] = {0, 0, 0};
-    double prof_ms[ST_PROF_NKIND] = {0, 0, 0};
-    long prof_n[ST_PROF_NKIND] = {0, 0, 0};
+    size_t evused[ST_PROF_NKIND] = {};
+    double prof_ms[ST_PROF_NKIND] = {};
+    long prof_n[ST_PROF_NKIND] = {};
 };
 
 static void prof_mark(st_ctx *c, int kind)
@@ -254,9 +254,11 @@ template <typename TS> static void smooth(st_ctx *c, int l, const TS *b, TS *x, 
     const float om = (float)c->prm.omega;
     const bool pr = (l == 0);
     for (int sw = 0; sw < nu; sw++) {
-        if (pr) prof_mark(c, ST_PROF_SMOOTH);
+        // the pass from x = 0 is another kernel (k_smooth0_pairs) with other bytes
+        const int k0 = (zero && sw == 0) ? ST_PROF_SMOOTH0 : ST_PROF_SMOOTH;
+        if (pr) prof_mark(c, k0);
         launch_smooth<TS>(g, c->fl[l], h.al[l], b, x, t, first, om, zero && sw == 0, c->s);
-        if (pr) { prof_mark(c, ST_PROF_SMOOTH); prof_mark(c, ST_PROF_SMOOTH); }
+        if (pr) { prof_mark(c, k0); prof_mark(c, ST_PROF_SMOOTH); }
         launch_smooth<TS>(g, c->fl[l], h.al[l], b, t, x, first ^ 1, om, false, c->s);
         if (pr) prof_mark(c, ST_PROF_SMOOTH);
     }
@@ -575,7 +577,8 @@ template <typename TS> static int mg_smooth_lowp(st_ctx *c, int l, int color, co
 {
     const Lvl &g = c->lv[l];
     const long nd = abi_dofs(c->dim, g.nx);
-    TS *bb = reinterpret_cast<TS *>(c->BA), *xi = reinterpret_cast<TS *>(c->Wv), *xo = reinterpret_cast<TS *>(c->LAM);
+    // scratch only: the held forward / adjoint solutions (X, LAM) are never touched
+    TS *bb = reinterpret_cast<TS *>(c->BA), *xi = reinterpret_cast<TS *>(c->Wv), *xo = reinterpret_cast<TS *>(c->V);
     CKR(stage_in(c, b, nd));
     launch_pack<TS>(g, c->abi_a, bb, 0, c->fl[l], 1.0, c->s);
     CKR(stage_in(c, xin, nd));
@@ -742,12 +745,16 @@ int st_get_params(const st_ctx *c, st_params *p)
 long st_num_dofs(const st_ctx *c) { return c ? c->ndof : -1; }
 long st_num_cells(const st_ctx *c) { return c ? c->ncell : -1; }
 
-int st_set_geometry(st_ctx *c, const uint8_t *fixed)
+int st_set_geometry(st_ctx *c, const uint8_t *fixed, int closed)
 {
-    if (!c || !fixed) return ST_EINVAL;
+    if (!c || !fixed || closed < -1 || closed > 1) return ST_EINVAL;
     CK(hipMemcpyAsync(c->abi_u8, fixed, c->ndof, hipMemcpyHostToDevice, c->s));
     launch_flags(c->lv[0], c->abi_u8, c->fl[0], c->s);
     CK(hipStreamSynchronize(c->s));
+    // the pressure gauge of the coarsest matrix belongs to the geometry, as in
+    // FluidSteadyState3D::init_boundary(..., bool _closed) (ST/FluidSteadyState3D.cpp:92-114);
+    // the coarsest inverse is rebuilt on the next solve (dirty)
+    if (closed >= 0) c->prm.closed = closed;
     c->geom = true;
     c->dirty = true;
     return ST_OK;
@@ -833,12 +840,15 @@ int st_profile_read(st_ctx *c, int kind, double *total_ms, long *launches, doubl
         //   residual            s(3 n_c + 4 n_f) + n_f/8   (x, b, alpha read, r write)
         //   smoother colour     half of a two-colour sweep s(3 n_c + 4 n_f) + n_f/8
         //                       (x, b, alpha read once, x written once per sweep)
+        //   zero-start pass     s(2 n_c + 3 n_f) + n_f/8  (b, alpha read, x written; x = 0
+        //                       is not read)
         // s: storage of the level-0 multigrid vectors; the Krylov operator of the FP32
         // path is FP32, that of the BF16 path FP64 on FP32 vectors
         const double nc = (double)c->ncell, nf = (double)(c->ndof - c->ncell);
         const int pr = c->prm.precision;
         const double sv = (pr == ST_FP32) ? 4.0 : (pr == ST_BF16 ? 2.0 : 8.0);
         if (kind == ST_PROF_SMOOTH) *bytes = 0.5 * (sv * (3.0 * nc + 4.0 * nf) + nf / 8.0);
+        else if (kind == ST_PROF_SMOOTH0) *bytes = sv * (2.0 * nc + 3.0 * nf) + nf / 8.0;
         else if (kind == ST_PROF_APPLY) {
             const double sx = (pr == ST_FP64) ? 8.0 : 4.0, sa = (pr == ST_FP32) ? 4.0 : 8.0, sy = sa;
             *bytes = (sx + sy) * (nc + nf) + sa * nf + nf / 8.0;
@@ -1101,8 +1111,8 @@ int st_apply(st_ctx *c, const double *x, double *y)
     if (!c->geom || !c->alpha) return fail(c, ST_ESTATE, "geometry and penalty/design must be set");
     CKR(stage_in(c, x, c->ndof));
     launch_pack<double>(c->lv[0], c->abi_a, c->Wv, 0, c->fl[0], 1.0, c->s);
-    launch_apply<double>(c->lv[0], c->fl[0], c->h64.al[0], c->Wv, c->LAM, c->s);
-    launch_unpack<double>(c->lv[0], c->LAM, c->abi_b, c->s);
+    launch_apply<double>(c->lv[0], c->fl[0], c->h64.al[0], c->Wv, c->BA, c->s);
+    launch_unpack<double>(c->lv[0], c->BA, c->abi_b, c->s);
     return stage_out(c, y, c->ndof);
 }
 
@@ -1111,8 +1121,8 @@ int st_apply_device(st_ctx *c, const double *d_x, double *d_y)
     if (!c || !d_x || !d_y) return ST_EINVAL;
     if (!c->geom || !c->alpha) return fail(c, ST_ESTATE, "geometry and penalty/design must be set");
     launch_pack<double>(c->lv[0], d_x, c->Wv, 0, c->fl[0], 1.0, c->s);
-    launch_apply<double>(c->lv[0], c->fl[0], c->h64.al[0], c->Wv, c->LAM, c->s);
-    launch_unpack<double>(c->lv[0], c->LAM, d_y, c->s);
+    launch_apply<double>(c->lv[0], c->fl[0], c->h64.al[0], c->Wv, c->BA, c->s);
+    launch_unpack<double>(c->lv[0], c->BA, d_y, c->s);
     CK(hipStreamSynchronize(c->s));
     return ST_OK;
 }
@@ -1484,9 +1494,9 @@ int st_mg_smooth(st_ctx *c, int l, int color, const double *b, const double *xin
     launch_pack<double>(g, c->abi_a, c->BA, 0, c->fl[l], 1.0, c->s);
     CKR(stage_in(c, xin, nd));
     launch_pack<double>(g, c->abi_a, c->Wv, 0, c->fl[l], 1.0, c->s);
-    launch_fill<double>((long)g.nfield * g.S, c->LAM, 0.0, c->s);
-    launch_smooth<double>(g, c->fl[l], c->h64.al[l], c->BA, c->Wv, c->LAM, color, (float)c->prm.omega, false, c->s);
-    launch_unpack<double>(g, c->LAM, c->abi_b, c->s);
+    launch_fill<double>((long)g.nfield * g.S, c->V, 0.0, c->s);
+    launch_smooth<double>(g, c->fl[l], c->h64.al[l], c->BA, c->Wv, c->V, color, (float)c->prm.omega, false, c->s);
+    launch_unpack<double>(g, c->V, c->abi_b, c->s);
     return stage_out(c, xout, nd);
 }
 

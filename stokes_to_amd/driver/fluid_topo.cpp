@@ -299,7 +299,6 @@ int main(int argc, char **argv)
     prm.q = P.q;
     prm.power = 3.0;
     prm.alpha_scale = 1.0;
-    prm.closed = 1;
     prm.filter_radius = a.filter;
     prm.rtol = a.rtol;
     prm.optimizer = a.opt == "oc" ? ST_OPT_OC : ST_OPT_MMA;
@@ -309,7 +308,9 @@ int main(int argc, char **argv)
     st_ctx *c = nullptr;
     int rc = st_create(&c, &prm);
     if (rc != ST_OK) die("st_create failed (" + std::to_string(rc) + "); is a GPU visible and -s = 4*2^k?");
-    check(c, st_set_geometry(c, P.fixed.data()), "st_set_geometry");
+    // test 0 is a closed box: the reference passes closed = true to init_boundary
+    // (FT/FluidTopoOptDriver3D.cpp:165)
+    check(c, st_set_geometry(c, P.fixed.data(), 1), "st_set_geometry");
     check(c, st_set_rhs(c, P.body.data(), nullptr), "st_set_rhs");
     std::vector<double> rho((size_t)P.nc, a.frac), x((size_t)P.tot);
     check(c, st_set_design(c, rho.data()), "st_set_design");

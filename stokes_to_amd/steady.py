@@ -65,10 +65,14 @@ class FluidSteadyState:
         return rc
 
     # ---- ST/FluidSteadyState3D.cpp:92-150 ----
-    def init_boundary(self, fixed):
+    def init_boundary(self, fixed, closed=None):
+        """fixed[cells | faces] (1 = fixed); closed: pressure gauge of a domain whose
+        boundary is fixed everywhere (the reference's init_boundary(..., _closed));
+        None keeps the context's value."""
         f = np.ascontiguousarray(fixed, dtype=np.uint8)
         assert f.size == self.ndof
-        self._check(self.lib.st_set_geometry(self.h, u8ptr(f)))
+        c = -1 if closed is None else int(bool(closed))
+        self._check(self.lib.st_set_geometry(self.h, u8ptr(f), c))
 
     def update_penalty(self, pen):
         pen = _f64(pen)

@@ -18,8 +18,9 @@ DRIVER = os.path.join(ROOT, "stokes_to_amd", "fluid_topo")
 def _test0_solver(make_solver, P, **kw):
     a = P.alpha
     s = make_solver(P.dim, P.n, alpha_max=a.alpha_max, alpha_min=a.alpha_min, q=a.q, power=a.power,
-                    alpha_scale=a.scale, closed=1, filter_radius=1.0, **kw)
-    s.init_boundary(P.fixed)
+                    alpha_scale=a.scale, filter_radius=1.0, **kw)
+    # the gauge comes with the geometry, as in init_boundary(..., closed=true)
+    s.init_boundary(P.fixed, closed=P.closed)
     s.update_b(body=P.body)
     return s
 
@@ -86,3 +87,45 @@ def test_fluid_topo_driver_end_to_end(tmp_path, driver, n):
     assert output.read_points(str(out / "0" / "psi_N")).shape == (nfix, 3)
     assert output.read_points(str(out / "0" / "psi_D")).shape == (0, 3)
     assert (out / "0" / "last_frame.txt").read_text() == str(lf - 1)
+
+
+BINDING = os.path.join(ROOT, "stokes_to_amd", "binding_test0")
+
+
+def test_reference_call_sequence_through_the_cpp_binding(tmp_path, orc):
+    """The C++ binding a maintainer adds to the reference (stokes_to_amd/driver/
+    hip_steady_state3d.h, shown in INTEGRATION.md) run in the reference's order --
+    init, init_boundary(..., closed = true), update_penalty, update_b, solve -- on the
+    reference's test 0 at 32^3, a closed box: the velocity and the pressure modulo its
+    mean agree with the oracle's solve."""
+    n = 32
+    P = reference_test0(3, n)
+    rho = np.random.default_rng(2).uniform(0.05, 1.0, n ** 3)
+    nn, sz = orc.sizes(3, *P.shape)
+    nc = sz[0]
+    fa = orc.design_to_alpha(3, P.shape, rho, P.alpha.array(), P.fixed[nc:])
+    d = tmp_path
+    P.fixed[:nc].astype(np.int32).tofile(d / "cell_fixed")
+    P.fixed[nc:].astype(np.int32).tofile(d / "face_fixed")
+    P.body[:nc].tofile(d / "cell_b")
+    P.body[nc:].tofile(d / "face_b")
+    np.zeros(nc).tofile(d / "cell_pen")
+    fa.tofile(d / "face_pen")
+    (d / "closed").write_text("1")
+    print(f"binding_test0 {n} (test 0, closed box)", flush=True)
+    r = subprocess.run([BINDING, str(n), str(d)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    iters, relres = r.stdout.split()
+    assert float(relres) <= 1e-10
+    x = np.concatenate([np.fromfile(d / "p"), np.fromfile(d / "vel")])
+    pen = np.zeros(nn)
+    pen[nc:] = fa
+    b = np.where(P.fixed.astype(bool), 0.0, P.body)
+    mg = orc.MG(3, P.shape, P.fixed, pen, closed=1)
+    xo, it, rr, _ = mg.fgmres(b, rtol=1e-12)
+    assert rel(x[nc:], xo[nc:]) < 1e-8
+    assert rel(x[:nc] - x[:nc].mean(), xo[:nc] - xo[:nc].mean()) < 1e-7
+    # without the gauge the same sequence stops with the library's error, not silently
+    (d / "closed").write_text("0")
+    r = subprocess.run([BINDING, str(n), str(d)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 1 and "singular" in r.stderr

@@ -243,3 +243,45 @@ def test_solve_is_bitwise_reproducible(make_solver, prec):
     assert st1["iters"] == st2["iters"]
     assert np.array_equal(x1, x2)
 
+
+
+@pytest.mark.parametrize("dim,n,prec,tol", [(3, 16, "fp64", 1e-12), (2, 32, "fp64", 1e-12), (3, 64, "fp64", 1e-12),
+                                            (3, 64, "fp32", 1e-6)])
+def test_smoother_with_fixed_cells_matches_oracle(make_solver, orc, dim, n, prec, tol):
+    """Fixed cells (a definite-fix pressure row) change the smoother block: the
+    curl-curl couplings between the cell's faces no longer cancel and the block is
+    solved by Sherman-Morrison (pinned against dense probed blocks in test_oracle.py).
+    16^3 / 2D: the per-cell kernel; 64^3: the tiled kernel (FP64 and FP32 storage)."""
+    P, rho = setup(dim, n)
+    nc = n ** dim
+    rng = np.random.default_rng(31)
+    P.fixed[rng.choice(nc, nc // 16, replace=False)] = 1
+    s = solver_for(make_solver, P, rho, precision=prec)
+    fixed, vol, pen = oracle_desc(orc, P, rho)
+    x = random_dofs(P, 13)
+    assert rel(s.applyMapping(x), orc.apply(dim, P.shape, fixed, vol, pen, x)) < 1e-12
+    b = random_dofs(P, 14, dirichlet=False)
+    for color in (0, 1):
+        xo = orc.smooth_color(dim, P.shape, fixed, pen, b, x, color, 1.0)
+        assert rel(s.mg_smooth(0, color, b, x, mixed=(prec != "fp64")), xo) < tol, color
+    if prec == "fp64":
+        x, st = s.solve()
+        assert st["converged"]
+        r = orc.lift_rhs(dim, P.shape, fixed, P.dirichlet) - orc.apply(dim, P.shape, fixed, vol, pen, x)
+        assert np.linalg.norm(r) <= 1.5e-8 * np.linalg.norm(orc.lift_rhs(dim, P.shape, fixed, P.dirichlet))
+
+
+@pytest.mark.parametrize("n", [16, 64])
+def test_2d_objective_is_the_reference_face_laplacian(make_solver, orc, n):
+    """2D objective and adjoint right-hand side against the literal restatement of the
+    reference's 2D FluidEnergyGrad (face 5-point Laplacian of the unmasked velocity,
+    FT/FluidEnergyGrad.cu:30-61), with Dirichlet inflow values on fixed faces."""
+    P, rho = setup(2, n)
+    s = solver_for(make_solver, P, rho)
+    fixed, vol, pen = oracle_desc(orc, P, rho)
+    for seed in (21, 22):
+        x = random_dofs(P, seed)
+        go = orc.energy_op_ref2d(P.shape, fixed, pen, x)
+        assert rel(s.compute_gradient_x(x), go) < 1e-12
+        Jo = 0.5 * x @ go
+        assert abs(s.compute_energy(x) - Jo) <= 1e-12 * abs(Jo)

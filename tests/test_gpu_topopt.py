@@ -99,3 +99,48 @@ def test_learned_mp_run_reaches_the_plain_design(make_solver):
             b.net_train(20, 1e-3)
     assert abs(ra["J"] - rb["J"]) <= 1e-6 * abs(ra["J"])
     assert rel(b.get_design(), a.get_design()) < 1e-4
+
+
+def test_config_128_learned_to_run_matches_plain_fp64():
+    """The 3D 128^3 configuration: channel problem, volume fraction 0.25, 60 OC
+    iterations, learned warm start trained online on (rho, u, p) pairs of the run's own
+    first 15 iterations, FP32 multigrid inside FP64 refinement, against an independent
+    plain FP64 run (zero starts, the reference's choice).  Every solve meets the 1e-8
+    true FP64 residual, so the runs agree: objectives within 1e-6 relative at every
+    iteration and final designs within 1e-4 relative L2."""
+    from stokes_to_amd.steady import FluidSteadyState
+    dim, n, iters, vf, train = 3, 128, 60, 0.25, 15
+    P = channel_problem(dim, n)
+    rho0 = np.full(n ** dim, vf)
+    runs = []
+    try:
+        for prec in ("fp64", "fp32"):
+            s = FluidSteadyState(dim, n, rtol=1e-8, precision=prec)
+            s.init_boundary(P.fixed)
+            s.set_design(rho0)
+            s.update_b(dirichlet=P.dirichlet)
+            runs.append(s)
+        a, b = runs
+        its_a, its_b = [], []
+        for it in range(iters):
+            ra = a.to_step(vf, init="zero")
+            rb = b.to_step(vf, init="learned")
+            assert ra["fwd"]["converged"] and rb["fwd"]["converged"] and ra["adj"]["converged"]
+            assert ra["fwd"]["relres"] <= 1e-8 and rb["fwd"]["relres"] <= 1e-8
+            assert abs(ra["J"] - rb["J"]) <= 1e-6 * abs(ra["J"]), (it, ra["J"], rb["J"])
+            its_a.append(ra["fwd"]["iters"])
+            its_b.append(rb["fwd"]["iters"])
+            if it < train:
+                b.record_sample()
+                b.net_train(3, 1e-3)
+            if it % 10 == 0:
+                print(f"128^3 it {it}: J {ra['J']:.9g} / {rb['J']:.9g}, forward iterations {its_a[-1]} / {its_b[-1]}",
+                      flush=True)
+        assert b.num_samples() == train
+        d = rel(b.get_design(), a.get_design())
+        print(f"128^3 final design rel L2 {d:.2e}; forward iterations plain median {np.median(its_a)}, "
+              f"learned+fp32 median {np.median(its_b[train:])}", flush=True)
+        assert d < 1e-4
+    finally:
+        for s in runs:
+            s.close()

@@ -151,6 +151,21 @@ def test_sensitivity_matches_finite_differences(orc, dim, n):
 
     J0, x, mg, (fixed, vol, pen) = J_of(rho)
     gx = orc.energy_op(dim, P.shape, fixed, vol, pen, x)
+    if dim == 2:
+        # the reference's 2D G (face Laplacian of the unmasked velocity, masked rows,
+        # FT/FluidEnergyGrad.cu:30-61) is not symmetric where a free face neighbours a
+        # Dirichlet face, so dJ/dx is (G + G^T) x / 2, and G^T x = G (masked x) on free
+        # rows.  The reference's adjoint right-hand side G x (FT/FluidTopoOpt.cpp:170-199)
+        # differs from it by L u_D / 2 on those rows only.
+        xm = np.where(fixed.astype(bool), 0.0, x)
+        gxm = orc.energy_op(dim, P.shape, fixed, vol, pen, xm)
+        free = ~fixed.astype(bool)
+        gs = np.where(free, 0.5 * (gx + gxm), gx)
+        d = gx - gs
+        nearD = _faces_next_to_dirichlet(P)
+        assert np.abs(d[free & ~nearD]).max() < 1e-13 * np.abs(gx).max()
+        assert np.abs(d[free & nearD]).max() > 1e-6 * np.abs(gx).max()
+        gx = gs
     lam, _, _, _ = mg.fgmres(gx, rtol=1e-14, maxit=300)  # A symmetric: adjoint = forward operator
     dj = orc.sensitivity(dim, P.shape, rho, am, P.fixed[nc:], x[nc:], lam[nc:])
     rng = np.random.default_rng(0)
@@ -167,6 +182,170 @@ def test_sensitivity_matches_finite_differences(orc, dim, n):
     for eps in (1e-3, 5e-4):
         errs.append(abs(J_of(rho + eps * d)[0] - J0 - eps * dj @ d))
     assert errs[1] < 0.3 * errs[0]
+
+
+def _faces_next_to_dirichlet(P):
+    """2D: free faces whose 5-point stencil (same component) touches a fixed face with a
+    nonzero Dirichlet value."""
+    n = P.n
+    nc = n * n
+    near = np.zeros(P.fixed.size, bool)
+    o = nc
+    for A, B in ((n + 1, n), (n, n + 1)):
+        m = A * B
+        dv = (P.dirichlet[o:o + m] != 0).reshape(A, B)
+        hit = np.zeros((A, B), bool)
+        hit[1:] |= dv[:-1]
+        hit[:-1] |= dv[1:]
+        hit[:, 1:] |= dv[:, :-1]
+        hit[:, :-1] |= dv[:, 1:]
+        near[o:o + m] = hit.reshape(-1)
+        o += m
+    return near
+
+
+def _tridiag(m):
+    return 2.0 * np.eye(m) - np.eye(m, k=1) - np.eye(m, k=-1)
+
+
+def test_2d_objective_is_the_kronecker_face_laplacian(orc):
+    """The reference's 2D objective operator (FT/FluidEnergyGrad.cu:30-61): per velocity
+    component the 5-point Laplacian with zero extension, i.e. the Kronecker sum
+    T_A (x) I_B + I_A (x) T_B of 1D second differences over the face array (A x B),
+    applied to the unmasked velocity, rows of fixed faces zeroed, plus pen * x."""
+    n = 6
+    P, rho = setup(2, n, sigma=1.0)
+    fixed, vol, pen = oracle_desc(orc, P, rho)
+    pen[: n * n] = 0.0
+    nd = fixed.size
+    G = np.zeros((nd, nd))
+    e = np.zeros(nd)
+    for j in range(nd):
+        e[:] = 0
+        e[j] = 1
+        G[:, j] = orc.energy_op_ref2d(P.shape, fixed, pen, e)
+    assert np.array_equal(G, np.column_stack([orc.energy_op(2, P.shape, fixed, vol, pen, c) for c in np.eye(nd)]))
+    ref = np.zeros((nd, nd))
+    o = n * n
+    for A, B in ((n + 1, n), (n, n + 1)):
+        m = A * B
+        ref[o:o + m, o:o + m] = np.kron(_tridiag(A), np.eye(B)) + np.kron(np.eye(A), _tridiag(B))
+        o += m
+    ref[fixed.astype(bool), :] = 0.0       # masked rows, unmasked columns
+    ref += np.diag(pen)
+    assert np.abs(G - ref).max() < 1e-14
+    # a fixed face with a value feeds its free neighbours (the columns are not masked)
+    fx = np.flatnonzero(fixed[n * n:]) + n * n
+    assert np.abs(G[:, fx]).sum() > 0 and np.all(G[fx][:, ~fixed.astype(bool)] == 0)
+
+
+def test_2d_objective_quadratic_field_has_constant_laplacian(orc):
+    """u_x(i, j) = a i^2 + b j^2 + c i j + d: 4u - sum of neighbours = -(2a + 2b) at every
+    interior free face (second differences of a quadratic are constant); u_y likewise."""
+    n = 8
+    P = channel_problem(2, n)
+    nc = n * n
+    fixed = np.zeros(P.fixed.size, np.uint8)
+    pen = np.zeros(P.fixed.size)
+    x = np.zeros(P.fixed.size)
+    a, b, c, d = 0.7, -1.3, 0.4, 2.0
+    I, J = np.meshgrid(np.arange(n + 1), np.arange(n), indexing="ij")
+    x[nc:nc + (n + 1) * n] = (a * I ** 2 + b * J ** 2 + c * I * J + d).reshape(-1)
+    I2, J2 = np.meshgrid(np.arange(n), np.arange(n + 1), indexing="ij")
+    x[nc + (n + 1) * n:] = (b * I2 ** 2 + a * J2 ** 2 - c * I2 * J2).reshape(-1)
+    y = orc.energy_op_ref2d(P.shape, fixed, pen, x)
+    yx = y[nc:nc + (n + 1) * n].reshape(n + 1, n)
+    yy = y[nc + (n + 1) * n:].reshape(n, n + 1)
+    assert np.allclose(yx[1:-1, 1:-1], -(2 * a + 2 * b), atol=1e-12)
+    assert np.allclose(yy[1:-1, 1:-1], -(2 * b + 2 * a), atol=1e-12)
+    # J = x.Gx/2 >= 0 for a velocity that vanishes on fixed faces (G is then a Gram form)
+    P2, rho = setup(2, n, sigma=1.0)
+    f2, v2, p2 = oracle_desc(orc, P2, rho)
+    z = np.random.default_rng(9).standard_normal(f2.size) * (1 - f2)
+    assert orc.objective(2, P2.shape, f2, v2, p2, z) > 0
+
+
+def _cell_block(dim, n, i, j, k):
+    """ABI indices of a cell's block: its pressure and its 2*dim faces."""
+    nz = n if dim == 3 else 1
+    nc = n * n * nz
+    dims = [(n + 1, n, nz), (n, n + 1, nz), (n, n, nz + 1)][:dim]
+    offs = [nc]
+    for sh in dims[:-1]:
+        offs.append(offs[-1] + int(np.prod(sh)))
+    idx = [(i * n + j) * nz + k]
+    for d, (A, B, C) in enumerate(dims):
+        for s in (0, 1):
+            f = [i, j, k]
+            f[d] += s
+            idx.append(offs[d] + (f[0] * B + f[1]) * C + f[2])
+    return idx
+
+
+@pytest.mark.parametrize("dim,n", [(3, 4), (3, 8), (2, 8), (2, 16)])
+def test_smoother_colour_step_is_the_dense_probed_block_solve(orc, dim, n):
+    """The reference smooths by probing A for each colour cell's block [p_c + its faces]
+    and solving it with batched LU (ST/StokeLike.cu:41-96, CX/BatchInverse.cuh:49-102).
+    Here the blocks are extracted from the dense operator (unit-vector probes), solved
+    with numpy.linalg.solve, and one colour step x += omega A_BB^-1 (b - A x)_B is
+    compared with the oracle's closed-form step.  The grid has fixed walls, the inflow,
+    fixed cells, free boundary faces (outflow) and penalties from 0 to 1e4."""
+    P = channel_problem(dim, n)
+    nz = n if dim == 3 else 1
+    nc = n * n * nz
+    fixed = P.fixed.copy()
+    rng = np.random.default_rng(n + dim)
+    fixed[rng.choice(nc, max(2, nc // 16), replace=False)] = 1           # fixed cells
+    nd = fixed.size
+    vol = np.where(np.arange(nd) >= nc, 1.0 - fixed, 0.0)
+    pen = np.zeros(nd)
+    pen[nc:] = 10.0 ** rng.uniform(-2, 4, nd - nc) * rng.integers(0, 2, nd - nc)
+    pen[fixed.astype(bool)] = 0.0
+    A = np.zeros((nd, nd))
+    e = np.zeros(nd)
+    for j in range(nd):
+        e[:] = 0
+        e[j] = 1
+        A[:, j] = orc.apply(dim, P.shape, fixed, vol, pen, e)
+    b = rng.standard_normal(nd)
+    xin = rng.standard_normal(nd)
+    r = b - A @ xin
+    for color in (0, 1):
+        for omega in (1.0, 0.7):
+            got = orc.smooth_color(dim, P.shape, fixed, pen, b, xin, color, omega)
+            want = xin.copy()
+            for i in range(n):
+                for j in range(n):
+                    for k in range(nz):
+                        if (i + j + k) % 2 != color:
+                            continue
+                        B = _cell_block(dim, n, i, j, k)
+                        want[B] += omega * np.linalg.solve(A[np.ix_(B, B)], r[B])
+            assert np.abs(got - want).max() < 1e-11 * max(1.0, np.abs(want).max()), (color, omega)
+
+
+@pytest.mark.parametrize("R", [1.0, 1.5])
+def test_oc_exponent_closed_form(orc, R):
+    """OC (Andreassen et al. 2011): rho_new = rho (-dc / (lambda dV))^eta where no bound is
+    active, so (rho_new / rho) / (-dc / dV)^eta is one constant (lambda^-eta) over the
+    cells; a wrong exponent or a dropped dV leaves a spread of the size of the dc spread.
+    dV = H^T (1/(n W)) is the gradient of the filtered mean (uniform 1/n without a filter)."""
+    dim, shape = 3, (6, 6, 6)
+    n = 216
+    rng = np.random.default_rng(11)
+    rho = rng.uniform(0.4, 0.6, n)
+    dc = -rng.uniform(0.8, 1.2, n)
+    dv = orc.filt(dim, shape, R, np.full(n, 1.0 / n), transpose=True)
+    if R == 1.0:
+        assert np.allclose(dv, 1.0 / n, rtol=1e-15)
+    vf = orc.filt(dim, shape, R, rho).mean()
+    for eta in (0.5, 0.3, 1.0):
+        rn, ch = orc.oc_update(dim, shape, R, rho, dc, vf, move=0.2, eta=eta, rho_min=1e-3)
+        assert np.all(rn > rho - 0.2) and np.all(rn < np.minimum(rho + 0.2, 1.0))   # no bound active
+        c = (rn / rho) / (-dc / dv) ** eta
+        assert np.ptp(c) < 1e-10 * c.mean(), eta
+        # the multiplier meets the filtered volume
+        assert abs(orc.filt(dim, shape, R, rn).mean() - vf) < 1e-10
 
 
 @pytest.mark.parametrize("dim,shape", [(3, (5, 4, 6)), (2, (7, 5, 1))])
