@@ -37,7 +37,7 @@ enum {
     ST_ENOCONV = -3,    /* Krylov iteration cap reached before rtol */
     ST_ESINGULAR = -4,  /* coarsest-level matrix is singular (closed domain without gauge) */
     ST_ESTATE = -5,     /* call order: geometry / penalty / rhs missing */
-    ST_ENAN = -6        /* NaN or Inf in an input or in the residual */
+    ST_ENAN = -6        /* NaN or Inf in an input, the residual or the coarsest-level matrix */
 };
 
 enum { ST_FP64 = 0, ST_FP32 = 1, ST_BF16 = 2 };        /* preconditioner precision */
@@ -60,6 +60,9 @@ typedef struct {
                             st_set_geometry's closed argument sets it) */
     double filter_radius;/* density filter radius in cells (<= 1 disables) */
     int optimizer;       /* ST_OPT_OC (default) or ST_OPT_MMA, the reference's optimizer */
+    int fused_sweep;     /* 1: FP32/BF16 smoothing sweeps of the tiled levels in one pass (both
+                            colours, bit-identical in FP32); 0 (default): one pass per colour,
+                            faster on MI355X (the one-pass kernel holds 126 KB of LDS) */
 } st_params;
 
 typedef struct {
@@ -74,7 +77,7 @@ typedef struct {
 } st_stats;
 
 /* defaults: the configs' material model (alpha_max 1e4, q 0.1, power 1, scale 1/n^2),
-   FP64, rtol 1e-8, FGMRES(30), V(2,2), omega 1, filter radius 1.5 */
+   FP64, rtol 1e-8, FGMRES(30), V(2,2), omega 1, filter radius 1.5, colour passes */
 void st_default_params(st_params *p, int dim, int n);
 
 /* replaces FluidSteadyState3D::init (ST/FluidSteadyState3D.cpp:12-90): builds the
@@ -170,9 +173,11 @@ int st_to_step(st_ctx *c, int init, double volfrac, double move, double eta, dou
                double *vol, double *change, st_stats *fwd, st_stats *adj);
 
 /* ---- kernel timing (HIP events around each launch on the context stream) ---- */
-enum { ST_PROF_SMOOTH = 0, ST_PROF_APPLY = 1, ST_PROF_RESIDUAL = 2, ST_PROF_SMOOTH0 = 3, ST_PROF_NKIND = 4 };
+enum { ST_PROF_SMOOTH = 0, ST_PROF_APPLY = 1, ST_PROF_RESIDUAL = 2, ST_PROF_SMOOTH0 = 3, ST_PROF_SWEEP = 4,
+       ST_PROF_NKIND = 5 };
 /* ST_PROF_SMOOTH: smoother colour passes with x given (k_march<., SMOOTH>);
-   ST_PROF_SMOOTH0: the first pass of a V-cycle, from x = 0 (k_smooth0_pairs) */
+   ST_PROF_SMOOTH0: the first pass of a V-cycle, from x = 0 (k_smooth0_pairs);
+   ST_PROF_SWEEP: one-pass two-colour sweeps (k_sweep, params.fused_sweep) */
 /* on != 0 starts recording the finest-level launches of each kind; 0 stops */
 int st_profile(st_ctx *c, int on);
 /* total device ms and launch count recorded for `kind` since st_profile(c, 1);

@@ -105,5 +105,10 @@ void march_residual_f32(const Lvl &, const uint8_t *, const double *al, const do
                         double *part, hipStream_t);
 // FP64 operator applied to an FP32-stored vector (x float, alpha and y double)
 void march_apply_mixed(const Lvl &, const uint8_t *, const double *al, const float *x, double *y, hipStream_t);
+// one red-black sweep (colour col0, then the other) in one pass (st_sweep.hip);
+// FP32 arithmetic, TS = float or bf16 storage, levels with march_ok
+template <typename TS>
+void march_sweep(const Lvl &, const uint8_t *, const TS *al, const TS *b, const TS *xin, TS *xout, int col0,
+                 float omega, hipStream_t);
 
 }  // namespace st

@@ -520,14 +520,26 @@ __global__ void k_probe(Lvl g, const uint8_t *fl, const TS *al, const long *dofs
     }
 }
 
-// in-place Gauss-Jordan inverse with partial pivoting, one workgroup
+// in-place Gauss-Jordan inverse with partial pivoting, one workgroup.  info: 0 ok,
+// k > 0 zero pivot at step k (singular), -1 a non-finite entry (nothing is inverted)
 __global__ void __launch_bounds__(1024) k_gauss_jordan(int n, double *A, int *perm, int *info)
 {
     __shared__ double sv[1024];
     __shared__ int si[1024];
-    __shared__ int piv_s;
+    __shared__ int piv_s, bad;
     const int tid = threadIdx.x, nt = blockDim.x;
-    if (tid == 0) *info = 0;
+    if (tid == 0) {
+        *info = 0;
+        bad = 0;
+    }
+    __syncthreads();
+    for (long q = tid; q < (long)n * n; q += nt)
+        if (!isfinite(A[q])) bad = 1;
+    __syncthreads();
+    if (bad) {  // the same value for every thread: the whole workgroup returns here
+        if (tid == 0) *info = -1;
+        return;
+    }
     for (int k = 0; k < n; k++) {
         double best = -1.0;
         int bi = k;

@@ -236,6 +236,8 @@ static int setup_hierarchy(st_ctx *c)
     hipEventElapsedTime(&ms, c->e2, e3);
     hipEventDestroy(e3);
     c->last_setup_ms = ms;
+    if (info < 0)
+        return fail(c, ST_ENAN, "non-finite value in the coarsest-level matrix (a NaN or Inf in the design or penalty)");
     if (info != 0)
         return fail(c, ST_ESINGULAR,
                     "coarsest-level matrix is singular at pivot " + std::to_string(info) +
@@ -253,6 +255,33 @@ template <typename TS> static void smooth(st_ctx *c, int l, const TS *b, TS *x, 
     TS *t = h.t[l];
     const float om = (float)c->prm.omega;
     const bool pr = (l == 0);
+    if constexpr (!std::is_same<TS, double>::value) {
+        if (c->prm.fused_sweep && march_ok(g)) {
+            // FP32 arithmetic on a tiled level: whole sweeps in one pass (st_sweep.hip),
+            // ping-pong x -> t -> x; a V-cycle's first sweep starts from x = 0 with the
+            // per-cell zero-start pass and one colour pass (it reads no x)
+            TS *src = x, *dst = t;
+            int sw = 0;
+            if (zero) {
+                if (pr) prof_mark(c, ST_PROF_SMOOTH0);
+                launch_smooth<TS>(g, c->fl[l], h.al[l], b, x, x, first, om, true, c->s);
+                if (pr) { prof_mark(c, ST_PROF_SMOOTH0); prof_mark(c, ST_PROF_SMOOTH); }
+                launch_smooth<TS>(g, c->fl[l], h.al[l], b, x, t, first ^ 1, om, false, c->s);
+                if (pr) prof_mark(c, ST_PROF_SMOOTH);
+                src = t;
+                dst = x;
+                sw = 1;
+            }
+            for (; sw < nu; sw++) {
+                if (pr) prof_mark(c, ST_PROF_SWEEP);
+                march_sweep<TS>(g, c->fl[l], h.al[l], b, src, dst, first, om, c->s);
+                if (pr) prof_mark(c, ST_PROF_SWEEP);
+                std::swap(src, dst);
+            }
+            if (src != x) launch_axpby<TS, TS>((long)g.nfield * g.S, 1.0, src, 0.0, x, c->s);
+            return;
+        }
+    }
     for (int sw = 0; sw < nu; sw++) {
         // the pass from x = 0 is another kernel (k_smooth0_pairs) with other bytes
         const int k0 = (zero && sw == 0) ? ST_PROF_SMOOTH0 : ST_PROF_SMOOTH;
@@ -611,6 +640,7 @@ void st_default_params(st_params *p, int dim, int n)
     p->closed = 0;
     p->filter_radius = 1.5;
     p->optimizer = ST_OPT_OC;
+    p->fused_sweep = 0;
 }
 
 int st_create(st_ctx **out, const st_params *p)
@@ -842,12 +872,14 @@ int st_profile_read(st_ctx *c, int kind, double *total_ms, long *launches, doubl
         //                       (x, b, alpha read once, x written once per sweep)
         //   zero-start pass     s(2 n_c + 3 n_f) + n_f/8  (b, alpha read, x written; x = 0
         //                       is not read)
+        //   one-pass sweep      s(3 n_c + 4 n_f) + n_f/8  (both colours)
         // s: storage of the level-0 multigrid vectors; the Krylov operator of the FP32
         // path is FP32, that of the BF16 path FP64 on FP32 vectors
         const double nc = (double)c->ncell, nf = (double)(c->ndof - c->ncell);
         const int pr = c->prm.precision;
         const double sv = (pr == ST_FP32) ? 4.0 : (pr == ST_BF16 ? 2.0 : 8.0);
         if (kind == ST_PROF_SMOOTH) *bytes = 0.5 * (sv * (3.0 * nc + 4.0 * nf) + nf / 8.0);
+        else if (kind == ST_PROF_SWEEP) *bytes = sv * (3.0 * nc + 4.0 * nf) + nf / 8.0;
         else if (kind == ST_PROF_SMOOTH0) *bytes = sv * (2.0 * nc + 3.0 * nf) + nf / 8.0;
         else if (kind == ST_PROF_APPLY) {
             const double sx = (pr == ST_FP64) ? 8.0 : 4.0, sa = (pr == ST_FP32) ? 4.0 : 8.0, sy = sa;

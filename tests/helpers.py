@@ -1,4 +1,6 @@
-"""Shared test setup: problems, oracle descriptors, random vectors."""
+"""Shared test setup: problems, oracle descriptors, random vectors, device buffers."""
+import ctypes
+
 import numpy as np
 
 from stokes_to_amd.problems import channel_problem, grf_density, dof_sizes
@@ -45,3 +47,51 @@ def solver_for(make_solver, P, rho=None, **kw):
 
 def sizes(dim, n):
     return dof_sizes(dim, n)
+
+
+class DeviceArray:
+    """A device buffer allocated through the HIP runtime that libstokes_hip.so uses
+    (the process's libamdhip64.so.7), for the entry points that take device pointers.
+    torch is not used for this: it ships its own HIP runtime under the same soname, and
+    a process can only hold one of them."""
+    _hip = None
+
+    @classmethod
+    def hip(cls):
+        if cls._hip is None:
+            from stokes_to_amd import _lib
+            _lib.load()
+            h = ctypes.CDLL("libamdhip64.so.7")
+            vp = ctypes.c_void_p
+            h.hipMalloc.argtypes = [ctypes.POINTER(vp), ctypes.c_size_t]
+            h.hipFree.argtypes = [vp]
+            h.hipMemcpy.argtypes = [vp, vp, ctypes.c_size_t, ctypes.c_int]
+            h.hipDeviceSynchronize.argtypes = []
+            h.hipStreamCreate.argtypes = [ctypes.POINTER(vp)]
+            h.hipStreamSynchronize.argtypes = [vp]
+            h.hipStreamDestroy.argtypes = [vp]
+            for f in ("hipMalloc", "hipFree", "hipMemcpy", "hipDeviceSynchronize", "hipStreamCreate",
+                      "hipStreamSynchronize", "hipStreamDestroy"):
+                getattr(h, f).restype = ctypes.c_int
+            cls._hip = h
+        return cls._hip
+
+    def __init__(self, host):
+        host = np.ascontiguousarray(host)
+        self.dtype, self.size = host.dtype, host.size
+        self.nbytes = max(host.nbytes, 1)
+        self.ptr = ctypes.c_void_p()
+        h = self.hip()
+        assert h.hipMalloc(ctypes.byref(self.ptr), self.nbytes) == 0
+        assert h.hipMemcpy(self.ptr, host.ctypes.data, host.nbytes, 1) == 0   # host to device
+
+    def get(self):
+        out = np.empty(self.size, self.dtype)
+        assert self.hip().hipDeviceSynchronize() == 0
+        assert self.hip().hipMemcpy(out.ctypes.data, self.ptr, out.nbytes, 2) == 0   # device to host
+        return out
+
+    def free(self):
+        if self.ptr:
+            self.hip().hipFree(self.ptr)
+            self.ptr = ctypes.c_void_p()

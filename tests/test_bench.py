@@ -4,6 +4,8 @@ import os
 import subprocess
 import sys
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
@@ -34,3 +36,25 @@ def test_roofline_record():
     assert "float" in r["kernel"]
     # the measured traffic is only known for the 256^3 workload it was taken on
     assert b.roofline_of(prof, 128, "fp64")["traffic"] is None
+
+
+def test_timed_iterations_cover_the_post_training_window():
+    """The TO run is always 80 iterations with 15 training iterations; K timed steps are
+    spread over iterations 15..79 whatever K is, so the headline does not depend on the
+    window the flags would otherwise pick."""
+    b = _bench()
+
+    class A:
+        pass
+    a = A()
+    a.steps, a.warmup = 65, 15
+    assert b.timed_iterations(a) == list(range(15, 80))
+    a.steps, a.warmup = 20, 5
+    t = b.timed_iterations(a)
+    assert len(t) == 20 and t[0] == 15 and t[-1] == 79 and t == sorted(set(t))
+    assert max(np.diff(t)) - min(np.diff(t)) <= 1
+    a.steps, a.warmup = 500, 0
+    assert b.timed_iterations(a) == list(range(15, 80))
+    a.steps, a.warmup = 10, 30                # more warmup than training: the window starts later
+    t = b.timed_iterations(a)
+    assert t[0] == 30 and t[-1] == 79 and len(t) == 10

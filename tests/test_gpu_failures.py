@@ -13,7 +13,7 @@ import numpy as np
 import pytest
 
 from conftest import rel
-from helpers import setup, solver_for
+from helpers import DeviceArray, setup, solver_for
 from stokes_to_amd import _lib
 from stokes_to_amd._lib import ST_ENAN, ST_ENOCONV, ST_ESINGULAR, StokesError, dptr
 from stokes_to_amd.problems import channel_problem, reference_test0
@@ -65,7 +65,6 @@ def test_closed_box_without_gauge_is_singular_then_solves_with_it(make_solver):
 
 
 def test_nan_inputs_return_enan_and_context_stays_usable(make_solver):
-    import torch
     P, rho = setup(3, 16)
     s = solver_for(make_solver, P, rho, rtol=1e-10)
     x0, st0 = s.solve()
@@ -82,9 +81,10 @@ def test_nan_inputs_return_enan_and_context_stays_usable(make_solver):
         s.update_b(body=body, dirichlet=P.dirichlet)
     assert e.value.code == ST_ENAN and "body" in str(e.value)
     # NaN in a device design: only the solve sees it (non-finite residual)
-    d = torch.from_numpy(bad).to("cuda")
-    ptr = ctypes.cast(ctypes.c_void_p(d.data_ptr()), ctypes.POINTER(ctypes.c_double))
+    d = DeviceArray(bad)
+    ptr = ctypes.cast(d.ptr, ctypes.POINTER(ctypes.c_double))
     assert s.lib.st_set_design_device(s.h, ptr) == 0
+    d.free()
     with pytest.raises(StokesError) as e:
         s.solve()
     assert e.value.code == ST_ENAN and "non-finite" in str(e.value)
@@ -99,7 +99,6 @@ def test_component_entry_points_leave_the_held_solutions_alone(make_solver):
     """st_apply, st_apply_device and st_mg_smooth(_mp) write through scratch buffers: a
     TO step that continues from the held forward and adjoint solutions (init keep)
     needs the same iterations and gives the same bits with those calls in between."""
-    import torch
     dim, n, vf = 3, 16, 0.25
     P = channel_problem(dim, n)
     rho0 = np.full(n ** dim, vf)
@@ -111,9 +110,11 @@ def test_component_entry_points_leave_the_held_solutions_alone(make_solver):
             rng = np.random.default_rng(3)
             x = rng.standard_normal(P.fixed.size)
             s.applyMapping(x)
-            dx = torch.from_numpy(x).to("cuda")
-            dy = torch.zeros_like(dx)
-            assert s.lib.st_apply_device(s.h, ctypes.c_void_p(dx.data_ptr()), ctypes.c_void_p(dy.data_ptr())) == 0
+            dx, dy = DeviceArray(x), DeviceArray(np.zeros_like(x))
+            assert s.lib.st_apply_device(s.h, dx.ptr, dy.ptr) == 0
+            assert rel(dy.get(), s.applyMapping(x)) == 0.0
+            dx.free()
+            dy.free()
             s.mg_smooth(0, 0, x, x)
             s.mg_smooth(0, 1, x, x, mixed=True)
         out.append(s.to_step(vf, init="keep"))
