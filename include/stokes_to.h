@@ -172,6 +172,19 @@ int st_mma_reset(st_ctx *c);
 int st_to_step(st_ctx *c, int init, double volfrac, double move, double eta, double rho_min, double *J,
                double *vol, double *change, st_stats *fwd, st_stats *adj);
 
+/* J = x^T G x / 2 of st_objective for callers that cannot allocate device memory
+   (code under HIP graph capture, serving loops, harnesses that forbid hipMalloc).
+   No counterpart in the reference.  No context: d_fixed (uint8, [cells|faces], 1 =
+   fixed), d_pen (FP64 [cells|faces]: zero cells, face alpha, zero on fixed faces) and
+   d_x (FP64 dof vector) are device arrays in the ABI layout; J is written to the
+   device scalar d_J, on `stream` (a hipStream_t; NULL = the null stream), without a
+   host synchronisation.  Scratch is a 16 KiB device array per GPU inside the library
+   (no allocation, no memset); calls that overlap on one GPU race on it.  n <= 1024,
+   dim 2 or 3 (G of the reference's 2D or 3D objective).  On error st_last_error(NULL)
+   gives the message. */
+int st_objective_noalloc(int dim, int n, const uint8_t *d_fixed, const double *d_pen, const double *d_x, double *d_J,
+                         void *stream);
+
 /* ---- kernel timing (HIP events around each launch on the context stream) ---- */
 enum { ST_PROF_SMOOTH = 0, ST_PROF_APPLY = 1, ST_PROF_RESIDUAL = 2, ST_PROF_SMOOTH0 = 3, ST_PROF_SWEEP = 4,
        ST_PROF_NKIND = 5 };

@@ -74,6 +74,7 @@ SIGNATURES = {
     "st_apply_device": (I, [C, C, C]),
     "st_objective": (I, [C, D, D]),
     "st_energy_grad": (I, [C, D, D]),
+    "st_objective_noalloc": (I, [I, I, C, C, C, C, C]),
     "st_sensitivity": (I, [C, D, D, D, D]),
     "st_filter": (I, [C, D, D, I]),
     "st_oc_update": (I, [C, D, D, F, F, F, F, D, D]),

@@ -13,7 +13,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libstokes_hip.so")
 DRIVER = os.path.join(HERE, "fluid_topo")
-SOURCES = ["st_kernels.hip", "st_march.hip", "st_solver.hip", "st_net.hip", "st_mma.hip", "st_sweep.hip"]
+SOURCES = ["st_kernels.hip", "st_march.hip", "st_solver.hip", "st_net.hip", "st_mma.hip", "st_noalloc.hip", "st_sweep.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("ST_OFFLOAD_ARCH", "gfx950")
 

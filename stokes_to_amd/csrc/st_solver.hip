@@ -24,6 +24,10 @@
 
 using namespace st;
 
+namespace st {
+const char *noctx_error();  // last failure of a context-free entry point (st_noalloc.hip)
+}
+
 namespace {
 
 template <typename TS> struct Hier {
@@ -764,7 +768,7 @@ void st_destroy(st_ctx *c)
     delete c;
 }
 
-const char *st_last_error(const st_ctx *c) { return c ? c->err.c_str() : "null context"; }
+const char *st_last_error(const st_ctx *c) { return c ? c->err.c_str() : st::noctx_error(); }
 
 int st_get_params(const st_ctx *c, st_params *p)
 {
