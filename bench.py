@@ -45,8 +45,8 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 # measured HBM traffic per launch of the 256^3 smoother colour pass: rocprofv3
 # FETCH_SIZE and WRITE_SIZE runs scaled by a 1 GiB read / write of known size
-# (profiles/k_march_smooth256_counters.md)
-SMOOTH256_TRAFFIC_BYTES = {"fp64": 2.761e9, "fp32": 1.472e9}
+# (profiles/solver_kernels256_hbm_counters.md)
+SMOOTH256_TRAFFIC_BYTES = {"fp64": 2.766e9, "fp32": 1.475e9}
 # the same for the one-pass two-colour sweep (filled in from its counter runs)
 SWEEP256_TRAFFIC_BYTES = {}
 METRIC = "ms per Stokes solve to 1e-8 rel residual at 3D 256³: plain FP64 MG vs learned+MP"
@@ -63,6 +63,8 @@ def parse():
     ap.add_argument("--mean", type=float, default=0.2, help="GRF mean (grf workload)")
     ap.add_argument("--rtol", type=float, default=1e-8)
     ap.add_argument("--mp", default="fp32", choices=["fp32", "bf16"])
+    ap.add_argument("--nu", type=int, nargs=2, default=(2, 2), metavar=("PRE", "POST"),
+                    help="smoothing sweeps of the V-cycle (all contexts)")
     ap.add_argument("--epochs", type=int, default=3)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -80,7 +82,7 @@ def log(args, *a):
 
 def new_ctx(args, prob, precision):
     from stokes_to_amd.steady import FluidSteadyState
-    s = FluidSteadyState(3, args.n, precision=precision, rtol=args.rtol)
+    s = FluidSteadyState(3, args.n, precision=precision, rtol=args.rtol, nu_pre=args.nu[0], nu_post=args.nu[1])
     s.init_boundary(prob.fixed)
     s.update_b(dirichlet=prob.dirichlet)
     return s
