@@ -1142,6 +1142,104 @@ __global__ void __launch_bounds__(256) k_multiaxpy(long n, TW *w, const TV *V, l
     }
 }
 
+// Four consecutive elements per lane in one 16-byte (float, double: two) or 8-byte
+// (bf16) load or store, as FP64 values.
+template <typename T> struct Quad;
+template <> struct Quad<double> {
+    __device__ static void ld(const double *p, long i, double *o)
+    {
+        const double2 a = reinterpret_cast<const double2 *>(p)[2 * i], b = reinterpret_cast<const double2 *>(p)[2 * i + 1];
+        o[0] = a.x; o[1] = a.y; o[2] = b.x; o[3] = b.y;
+    }
+    __device__ static void st(double *p, long i, const double *o)
+    {
+        reinterpret_cast<double2 *>(p)[2 * i] = make_double2(o[0], o[1]);
+        reinterpret_cast<double2 *>(p)[2 * i + 1] = make_double2(o[2], o[3]);
+    }
+};
+template <> struct Quad<float> {
+    __device__ static void ld(const float *p, long i, double *o)
+    {
+        const float4 a = reinterpret_cast<const float4 *>(p)[i];
+        o[0] = a.x; o[1] = a.y; o[2] = a.z; o[3] = a.w;
+    }
+    __device__ static void st(float *p, long i, const double *o)
+    {
+        reinterpret_cast<float4 *>(p)[i] = make_float4((float)o[0], (float)o[1], (float)o[2], (float)o[3]);
+    }
+};
+template <> struct Quad<bf16> {
+    __device__ static void ld(const bf16 *p, long i, double *o)
+    {
+        const uint2 a = reinterpret_cast<const uint2 *>(p)[i];
+        bf16 h[4] = {{(uint16_t)(a.x & 0xffff)}, {(uint16_t)(a.x >> 16)}, {(uint16_t)(a.y & 0xffff)}, {(uint16_t)(a.y >> 16)}};
+        for (int e = 0; e < 4; e++) o[e] = to_f(h[e]);
+    }
+    __device__ static void st(bf16 *p, long i, const double *o)
+    {
+        uint16_t h[4];
+        for (int e = 0; e < 4; e++) h[e] = to_bf16((float)o[e]).v;
+        reinterpret_cast<uint2 *>(p)[i] = make_uint2(h[0] | ((uint32_t)h[1] << 16), h[2] | ((uint32_t)h[3] << 16));
+    }
+};
+
+// k_multidot / k_multiaxpy on four elements per lane (n and ld multiples of 4, the
+// vectors 16-byte aligned): one wide load per vector and group instead of four
+// narrow ones; n4 = n / 4 groups
+template <typename TW, typename TV, int M>
+__global__ void __launch_bounds__(256) k_multidot4(long n4, const TW *w, const TV *V, long ld, int m, double *part)
+{
+    __shared__ double sh[16];
+    double acc[M];
+#pragma unroll
+    for (int v = 0; v < M; v++) acc[v] = 0.0;
+    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n4; q += (long)gridDim.x * blockDim.x) {
+        double a[4];
+        Quad<TW>::ld(w, q, a);
+#pragma unroll
+        for (int v = 0; v < M; v++)
+            if (v < m) {
+                double b[4];
+                Quad<TV>::ld(V + (long)v * ld, q, b);
+                acc[v] += (a[0] * b[0] + a[1] * b[1]) + (a[2] * b[2] + a[3] * b[3]);
+            }
+    }
+    for (int v = 0; v < m; v++) {
+        double s = block_sum(acc[v], sh);
+        if (threadIdx.x == 0) part[(long)v * gridDim.x + blockIdx.x] = s;
+    }
+}
+
+template <typename TW, typename TV, int M>
+__global__ void __launch_bounds__(256) k_multiaxpy4(long n4, TW *w, const TV *V, long ld, int m, const double *h,
+                                                    const double *sc, double *part)
+{
+    __shared__ double sh[16];
+    __shared__ double hs[64];
+    if (threadIdx.x < (unsigned)m) hs[threadIdx.x] = sc ? h[threadIdx.x] * sc[threadIdx.x] : h[threadIdx.x];
+    __syncthreads();
+    double acc = 0.0;
+    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n4; q += (long)gridDim.x * blockDim.x) {
+        double a[4];
+        Quad<TW>::ld(w, q, a);
+#pragma unroll
+        for (int v = 0; v < M; v++)
+            if (v < m) {
+                double b[4];
+                Quad<TV>::ld(V + (long)v * ld, q, b);
+                const double hv = hs[v];
+#pragma unroll
+                for (int e = 0; e < 4; e++) a[e] -= hv * b[e];
+            }
+        Quad<TW>::st(w, q, a);
+        acc += (a[0] * a[0] + a[1] * a[1]) + (a[2] * a[2] + a[3] * a[3]);
+    }
+    if (part) {
+        double s = block_sum(acc, sh);
+        if (threadIdx.x == 0) part[blockIdx.x] = s;
+    }
+}
+
 // y = a*x + b*y  (types may differ: conversions between precisions)
 template <typename TX, typename TY>
 __global__ void __launch_bounds__(256) k_axpby(long n, double a, const TX *x, double b, TY *y)
@@ -1323,9 +1421,24 @@ void launch_dot(long n, const TA *a, const TB *b, double *part, hipStream_t s)
 {
     hipLaunchKernelGGL((k_dot<TA, TB>), dim3(nb_grid(n)), dim3(256), 0, s, n, a, b, part);
 }
+// the four-element kernels apply when every vector starts on a 16-byte boundary
+template <typename TW, typename TV>
+static bool quad_ok(long n, const TW *w, const TV *V, long ld)
+{
+    return n % 4 == 0 && ld % 4 == 0 && ((uintptr_t)w % 16) == 0 && ((uintptr_t)V % 16) == 0 &&
+           (ld * (long)sizeof(TV)) % 16 == 0;
+}
+
 template <typename TW, typename TV>
 void launch_multidot(long n, const TW *w, const TV *V, long ld, int m, double *part, hipStream_t s)
 {
+    if (quad_ok(n, w, V, ld)) {
+        const dim3 gr(nb_grid(n));
+        if (m <= 8) hipLaunchKernelGGL((k_multidot4<TW, TV, 8>), gr, dim3(256), 0, s, n / 4, w, V, ld, m, part);
+        else if (m <= 16) hipLaunchKernelGGL((k_multidot4<TW, TV, 16>), gr, dim3(256), 0, s, n / 4, w, V, ld, m, part);
+        else hipLaunchKernelGGL((k_multidot4<TW, TV, 32>), gr, dim3(256), 0, s, n / 4, w, V, ld, m, part);
+        return;
+    }
     if (m <= 8) hipLaunchKernelGGL((k_multidot<TW, TV, 8>), dim3(nb_grid(n)), dim3(256), 0, s, n, w, V, ld, m, part);
     else if (m <= 16) hipLaunchKernelGGL((k_multidot<TW, TV, 16>), dim3(nb_grid(n)), dim3(256), 0, s, n, w, V, ld, m, part);
     else hipLaunchKernelGGL((k_multidot<TW, TV, 32>), dim3(nb_grid(n)), dim3(256), 0, s, n, w, V, ld, m, part);
@@ -1334,6 +1447,13 @@ template <typename TW, typename TV>
 void launch_multiaxpy(long n, TW *w, const TV *V, long ld, int m, const double *h, const double *sc, double *part,
                       hipStream_t s)
 {
+    if (quad_ok(n, (const TW *)w, V, ld)) {
+        const dim3 gr(nb_grid(n));
+        if (m <= 8) hipLaunchKernelGGL((k_multiaxpy4<TW, TV, 8>), gr, dim3(256), 0, s, n / 4, w, V, ld, m, h, sc, part);
+        else if (m <= 16) hipLaunchKernelGGL((k_multiaxpy4<TW, TV, 16>), gr, dim3(256), 0, s, n / 4, w, V, ld, m, h, sc, part);
+        else hipLaunchKernelGGL((k_multiaxpy4<TW, TV, 32>), gr, dim3(256), 0, s, n / 4, w, V, ld, m, h, sc, part);
+        return;
+    }
     if (m <= 8) hipLaunchKernelGGL((k_multiaxpy<TW, TV, 8>), dim3(nb_grid(n)), dim3(256), 0, s, n, w, V, ld, m, h, sc, part);
     else if (m <= 16) hipLaunchKernelGGL((k_multiaxpy<TW, TV, 16>), dim3(nb_grid(n)), dim3(256), 0, s, n, w, V, ld, m, h, sc, part);
     else hipLaunchKernelGGL((k_multiaxpy<TW, TV, 32>), dim3(nb_grid(n)), dim3(256), 0, s, n, w, V, ld, m, h, sc, part);
