@@ -698,42 +698,37 @@ template <typename TS, int DIM>
 __global__ void __launch_bounds__(512) k_vcycle_tail_lds(const TailArgs<TS> *__restrict__ a, const TS *b0g, TS *x0g)
 {
     __shared__ __align__(16) unsigned char pool[ST_TAIL_LDS_BYTES];
-    __shared__ const uint8_t *Pfl[ST_MAX_LEVELS];
-    __shared__ const TS *Pal[ST_MAX_LEVELS];
-    __shared__ TS *Pb[ST_MAX_LEVELS], *Px[ST_MAX_LEVELS], *Pt[ST_MAX_LEVELS], *Pr[ST_MAX_LEVELS];
     const long T = blockDim.x;
     const int l0 = a->l0, L = a->L;
     const float om = a->omega;
-    if (threadIdx.x == 0) {
-        for (int l = l0; l < L; l++) {
-            Pfl[l] = (const uint8_t *)(pool + a->lds.fl[l]);
-            Pal[l] = (const TS *)(pool + a->lds.al[l]);
-            Pb[l] = (TS *)(pool + a->lds.b[l]);
-            Px[l] = (TS *)(pool + a->lds.x[l]);
-            Pt[l] = (TS *)(pool + a->lds.t[l]);
-            Pr[l] = (TS *)(pool + a->lds.r[l]);
-        }
-    }
+    // level buffers in the pool, formed where they are used (so the compiler sees LDS
+    // pointers and emits LDS instructions, not flat ones)
+    auto Pfl = [&](int l) { return (uint8_t *)(pool + a->lds.fl[l]); };
+    auto Pal = [&](int l) { return (TS *)(pool + a->lds.al[l]); };
+    auto Pb = [&](int l) { return (TS *)(pool + a->lds.b[l]); };
+    auto Px = [&](int l) { return (TS *)(pool + a->lds.x[l]); };
+    auto Pt = [&](int l) { return (TS *)(pool + a->lds.t[l]); };
+    auto Pr = [&](int l) { return (TS *)(pool + a->lds.r[l]); };
     for (long q = threadIdx.x; q < a->lds.bytes / 4; q += T) ((unsigned *)pool)[q] = 0u;  // padding = 0
     __syncthreads();
     for (int l = l0; l < L; l++) {
         const Lvl g = a->g[l];
-        uint8_t *fl = (uint8_t *)Pfl[l];
-        TS *al = (TS *)Pal[l];
+        uint8_t *fl = (uint8_t *)Pfl(l);
+        TS *al = (TS *)Pal(l);
         for (long q = threadIdx.x; q < g.S; q += T) fl[q] = a->fl[l][q];
         for (long q = threadIdx.x; q < (long)g.dim * g.S; q += T) al[q] = a->al[l][q];
     }
     {
         const Lvl g = a->g[l0];
         const TS *bsrc = (l0 == 0) ? b0g : a->b[l0];
-        for (long q = threadIdx.x; q < (long)g.nfield * g.S; q += T) Pb[l0][q] = bsrc[q];
+        for (long q = threadIdx.x; q < (long)g.nfield * g.S; q += T) Pb(l0)[q] = bsrc[q];
     }
     __syncthreads();
     auto smooth = [&](int l, int nu, int first, bool zero) {
         const Lvl g = a->g[l];
-        const uint8_t *fl = Pfl[l];
-        const TS *al = Pal[l], *b = Pb[l];
-        TS *x = Px[l], *t = Pt[l];
+        const uint8_t *fl = Pfl(l);
+        const TS *al = Pal(l), *b = Pb(l);
+        TS *x = Px(l), *t = Pt(l);
         const long nc = (long)g.nx * g.ny * g.nz;
         for (int sw = 0; sw < nu; sw++) {
             if (zero && sw == 0)
@@ -750,37 +745,37 @@ __global__ void __launch_bounds__(512) k_vcycle_tail_lds(const TailArgs<TS> *__r
         if (a->nu_pre > 0) {
             smooth(l, a->nu_pre, 0, true);
         } else {
-            for (long q = threadIdx.x; q < (long)g.nfield * g.S; q += T) Px[l][q] = Cvt<TS>::st((typename Cvt<TS>::C)0);
+            for (long q = threadIdx.x; q < (long)g.nfield * g.S; q += T) Px(l)[q] = Cvt<TS>::st((typename Cvt<TS>::C)0);
             __syncthreads();
         }
-        for (long n = threadIdx.x; n < g.S; n += T) residual_body<TS, DIM>(g, Pfl[l], Pal[l], Pb[l], Px[l], Pr[l], n);
+        for (long n = threadIdx.x; n < g.S; n += T) residual_body<TS, DIM>(g, Pfl(l), Pal(l), Pb(l), Px(l), Pr(l), n);
         __syncthreads();
         const Lvl gc = a->g[l + 1];
-        for (long n = threadIdx.x; n < gc.S; n += T) restrict_body<TS, DIM>(g, gc, Pfl[l + 1], Pr[l], Pb[l + 1], n);
+        for (long n = threadIdx.x; n < gc.S; n += T) restrict_body<TS, DIM>(g, gc, Pfl(l + 1), Pr(l), Pb(l + 1), n);
         __syncthreads();
     }
     const int Lc = L - 1, nd = a->nd;
     double *bs = (double *)(pool + a->lds.bs);
     const long *dofs = a->dofs;
-    for (int c = threadIdx.x; c < nd; c += T) bs[c] = (double)Cvt<TS>::ld(Pb[Lc][dofs[c]]);
+    for (int c = threadIdx.x; c < nd; c += T) bs[c] = (double)Cvt<TS>::ld(Pb(Lc)[dofs[c]]);
     __syncthreads();
     for (int r = threadIdx.x; r < nd; r += T) {
         double sacc = 0.0;
         const double *row = a->minv + (long)r * nd;
         for (int c = 0; c < nd; c++) sacc += row[c] * bs[c];
-        Px[Lc][dofs[r]] = Cvt<TS>::st((typename Cvt<TS>::C)sacc);
+        Px(Lc)[dofs[r]] = Cvt<TS>::st((typename Cvt<TS>::C)sacc);
     }
     __syncthreads();
     for (int l = L - 2; l >= l0; l--) {
         const Lvl g = a->g[l];
-        for (long n = threadIdx.x; n < g.S; n += T) prolong_body<TS, DIM>(g, a->g[l + 1], Pfl[l], Px[l + 1], Px[l], n);
+        for (long n = threadIdx.x; n < g.S; n += T) prolong_body<TS, DIM>(g, a->g[l + 1], Pfl(l), Px(l + 1), Px(l), n);
         __syncthreads();
         if (a->nu_post > 0) smooth(l, a->nu_post, 1, false);
     }
     {
         const Lvl g = a->g[l0];
         TS *xdst = (l0 == 0) ? x0g : a->x[l0];
-        for (long q = threadIdx.x; q < (long)g.nfield * g.S; q += T) xdst[q] = Px[l0][q];
+        for (long q = threadIdx.x; q < (long)g.nfield * g.S; q += T) xdst[q] = Px(l0)[q];
     }
 }
 
