@@ -33,7 +33,16 @@ struct Sten {
         if (DIM == 2) return (unsigned)i <= (unsigned)g.nx && (unsigned)j <= (unsigned)g.ny && k == 0;
         return (unsigned)i <= (unsigned)g.nx && (unsigned)j <= (unsigned)g.ny && (unsigned)k <= (unsigned)g.nz;
     }
-    __device__ __forceinline__ int F(int i, int j, int k) const { return in(i, j, k) ? (int)fl[id(i, j, k)] : 0; }
+    // Flag and masked-velocity reads are branch-free: a node outside the box reads
+    // node 0 and the value is discarded, so the loads of a stencil do not wait for the
+    // flag and range tests and can be in flight at once (the gather kernels of the
+    // small levels are latency-bound).
+    __device__ __forceinline__ long idc(int i, int j, int k) const { return in(i, j, k) ? id(i, j, k) : 0; }
+    __device__ __forceinline__ int F(int i, int j, int k) const
+    {
+        const int f = (int)fl[idc(i, j, k)];
+        return in(i, j, k) ? f : 0;
+    }
     __device__ __forceinline__ C ldx(long idx) const
     {
         if (MODE == 1) return (idx == hot) ? C(1) : C(0);
@@ -43,14 +52,22 @@ struct Sten {
     {
         return in(i, j, k) ? ldx((long)f * g.S + id(i, j, k)) : C(0);
     }
+    // X with the load issued whatever the range test gives (the restriction's 12-point
+    // gathers; the prolongation's two-point ones are faster with the branch)
+    __device__ __forceinline__ C Xb(int f, int i, int j, int k) const
+    {
+        const C v = ldx((long)f * g.S + idc(i, j, k));
+        return in(i, j, k) ? v : C(0);
+    }
     // masked velocity w = m u
     __device__ __forceinline__ C W(int d, int i, int j, int k) const
     {
         if (DIM == 2 && d == 2) return C(0);
+        const long q = idc(i, j, k);
+        const C v = ldx((long)(1 + d) * g.S + q);
         if (!in(i, j, k)) return C(0);
-        long q = id(i, j, k);
         if (MODE != 2 && !((fl[q] >> (1 + d)) & 1)) return C(0);
-        return ldx((long)(1 + d) * g.S + q);
+        return v;
     }
     __device__ __forceinline__ C dv(int i, int j, int k) const
     {
@@ -370,7 +387,7 @@ __device__ __forceinline__ void restrict_body(const Lvl &gf, const Lvl &gc, cons
     if (fb & 1) {
         for (int a = 0; a < 2; a++)
             for (int b = 0; b < 2; b++)
-                for (int c = 0; c < kz; c++) sp += f.X(0, 2 * I + a, 2 * J + b, kz * K + c);
+                for (int c = 0; c < kz; c++) sp += f.Xb(0, 2 * I + a, 2 * J + b, kz * K + c);
         sp *= cp;
     }
     bc[n] = Cvt<TS>::st(sp);
@@ -386,7 +403,7 @@ __device__ __forceinline__ void restrict_body(const Lvl &gf, const Lvl &gc, cons
                         if (d == 0) { fi = 2 * I + on; fj = 2 * J + a; fk = (DIM == 3) ? 2 * K + b : 0; }
                         else if (d == 1) { fi = 2 * I + a; fj = 2 * J + on; fk = (DIM == 3) ? 2 * K + b : 0; }
                         else { fi = 2 * I + a; fj = 2 * J + b; fk = 2 * K + on; }
-                        su += w * f.X(1 + d, fi, fj, fk);
+                        su += w * f.Xb(1 + d, fi, fj, fk);
                     }
             }
             su *= cu;
