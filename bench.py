@@ -106,6 +106,7 @@ def summary(stats):
         "relres_max": float(max(st["relres"] for st in stats)),
         "converged": all(bool(st["converged"]) for st in stats),
         "setup_ms_median": float(np.median([st["setup_ms"] for st in stats])),
+        "init_ms_median": float(np.median([st["init_ms"] for st in stats])),
     }
 
 

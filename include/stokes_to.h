@@ -63,6 +63,10 @@ typedef struct {
     int fused_sweep;     /* 1: FP32/BF16 smoothing sweeps of the tiled levels in one pass (both
                             colours, bit-identical in FP32); 0 (default): one pass per colour,
                             faster on MI355X (the one-pass kernel holds 126 KB of LDS) */
+    int warm_order;      /* ST_INIT_KEEP / ST_INIT_LEARNED starts of a sequence of solves
+                            (st_to_step, repeated st_solve): 0 the held solution; 1 its linear,
+                            2 (default) its quadratic extrapolation from the previous solutions of
+                            the sequence, taken when its true FP64 residual is lower */
 } st_params;
 
 typedef struct {
@@ -72,7 +76,8 @@ typedef struct {
     double relres0;      /* true relative residual of the initial guess */
     double ms;           /* wall time of the solve incl. hierarchy setup (hipEvents) */
     double setup_ms;     /* hierarchy setup part (coarsening + coarsest inverse) */
-    double init_ms;      /* initial-guess part (learned predictor inference and its residual check) */
+    double init_ms;      /* initial-guess part (extrapolation, learned predictor inference and
+                            their residual checks) */
     int learned_used;    /* ST_INIT_LEARNED: 1 if the network's candidate had the smaller true residual */
 } st_stats;
 

@@ -28,7 +28,7 @@ class StParams(ctypes.Structure):
         ("precision", ctypes.c_int), ("rtol", ctypes.c_double), ("max_iter", ctypes.c_int),
         ("restart", ctypes.c_int), ("nu_pre", ctypes.c_int), ("nu_post", ctypes.c_int),
         ("omega", ctypes.c_double), ("closed", ctypes.c_int), ("filter_radius", ctypes.c_double),
-        ("optimizer", ctypes.c_int), ("fused_sweep", ctypes.c_int),
+        ("optimizer", ctypes.c_int), ("fused_sweep", ctypes.c_int), ("warm_order", ctypes.c_int),
     ]
 
 

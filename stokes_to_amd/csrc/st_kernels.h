@@ -88,6 +88,7 @@ template <typename TW, typename TV>
 void launch_multiaxpy(long n, TW *w, const TV *V, long ld, int m, const double *h, const double *sc, double *part,
                       hipStream_t);
 template <typename TX, typename TY> void launch_axpby(long n, double a, const TX *x, double b, TY *y, hipStream_t);
+void launch_lincomb3(long n, double a, const double *x, double b, const double *w, double c, double *y, hipStream_t);
 template <typename TS> void launch_fill(long n, TS *y, double v, hipStream_t);
 
 // 2.5D marching kernels (st_march.hip) for 3D levels with ny % 4 == 0, nz % 64 == 0

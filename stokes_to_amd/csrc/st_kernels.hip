@@ -1263,6 +1263,17 @@ __global__ void __launch_bounds__(256) k_axpby(long n, double a, const TX *x, do
     }
 }
 
+// y = a x + b w + c y (FP64; c = 0 does not read y): the extrapolated start of a solve
+__global__ void __launch_bounds__(256) k_lincomb3(long n, double a, const double *x, double b, const double *w,
+                                                  double c, double *y)
+{
+    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+        double v = a * x[q] + b * w[q];
+        if (c != 0.0) v += c * y[q];
+        y[q] = v;
+    }
+}
+
 template <typename TS>
 __global__ void __launch_bounds__(256) k_fill(long n, TS *y, double v)
 {
@@ -1469,6 +1480,10 @@ void launch_multiaxpy(long n, TW *w, const TV *V, long ld, int m, const double *
     if (m <= 8) hipLaunchKernelGGL((k_multiaxpy<TW, TV, 8>), dim3(nb_grid(n)), dim3(256), 0, s, n, w, V, ld, m, h, sc, part);
     else if (m <= 16) hipLaunchKernelGGL((k_multiaxpy<TW, TV, 16>), dim3(nb_grid(n)), dim3(256), 0, s, n, w, V, ld, m, h, sc, part);
     else hipLaunchKernelGGL((k_multiaxpy<TW, TV, 32>), dim3(nb_grid(n)), dim3(256), 0, s, n, w, V, ld, m, h, sc, part);
+}
+void launch_lincomb3(long n, double a, const double *x, double b, const double *w, double c, double *y, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_lincomb3, dim3(nb_grid(n)), dim3(256), 0, s, n, a, x, b, w, c, y);
 }
 template <typename TX, typename TY>
 void launch_axpby(long n, double a, const TX *x, double b, TY *y, hipStream_t s)

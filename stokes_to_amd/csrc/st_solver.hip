@@ -81,6 +81,11 @@ struct st_ctx {
     bool have_rhop = false, have_lam = false, have_grad = false;
     int learned_rejects = 0;  // consecutive rejections of the network's candidate
     long learned_calls = 0;
+    // previous solutions of the current sequence of forward (XH) and adjoint (LH) solves,
+    // for the extrapolated start: [0] = the one before the held solution, [1] = the one
+    // before that; nxh / nlh of them valid
+    double *XH[2] = {nullptr, nullptr}, *LH[2] = {nullptr, nullptr};
+    int nxh = 0, nlh = 0;
     // kernel timing: event pairs around finest-level launches
     bool prof = false;
     std::vector<hipEvent_t> evpool[ST_PROF_NKIND];
@@ -645,6 +650,7 @@ void st_default_params(st_params *p, int dim, int n)
     p->filter_radius = 1.5;
     p->optimizer = ST_OPT_OC;
     p->fused_sweep = 0;
+    p->warm_order = 2;
 }
 
 int st_create(st_ctx **out, const st_params *p)
@@ -661,6 +667,7 @@ int st_create(st_ctx **out, const st_params *p)
     }
     if (p->restart < 1 || p->restart > 32 || p->max_iter < 1 || !(p->rtol > 0)) return ST_EINVAL;
     if (p->precision < ST_FP64 || p->precision > ST_BF16) return ST_EINVAL;
+    if (p->warm_order < 0 || p->warm_order > 2) return ST_EINVAL;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return ST_EHIP;
     st_ctx *c = new st_ctx();
@@ -791,6 +798,7 @@ int st_set_geometry(st_ctx *c, const uint8_t *fixed, int closed)
     if (closed >= 0) c->prm.closed = closed;
     c->geom = true;
     c->dirty = true;
+    c->nxh = c->nlh = 0;
     return ST_OK;
 }
 
@@ -923,6 +931,7 @@ int st_set_rhs(st_ctx *c, const double *body, const double *dirichlet)
     launch_lift(g, c->fl[0], c->XD, c->BODY, c->B, c->s);
     CK(hipStreamSynchronize(c->s));
     c->rhs = true;
+    c->nxh = c->nlh = 0;
     return ST_OK;
 }
 
@@ -946,6 +955,58 @@ static int solve_core(st_ctx *c, const double *b, double *x, const double *xd, i
     prof_collect(c);
     if (st) *st = local;
     return rc;
+}
+
+// ------------------------------------------------- extrapolated start
+
+// Start of a solve that continues a sequence (ST_INIT_KEEP, ST_INIT_LEARNED): *xp holds
+// the previous solution x1, h[0] = x2 and h[1] = x3 the ones before it (nh valid).  The
+// candidate is the polynomial extrapolation of order min(nh, warm_order) through them,
+// 2 x1 - x2 or 3 x1 - 3 x2 + x3 (the designs of an optimisation change smoothly, and so
+// do their solutions), kept when its true FP64 residual is below that of x1.  After the
+// call *xp is the start and h[0] = x1, h[1] = x2: the three buffers are rotated; x1 is
+// copied only when the candidate loses (or when there is nothing to extrapolate yet).
+static int resid_norm(st_ctx *c, const double *b, const double *x, double *nrm);
+
+static int warm_extrapolate(st_ctx *c, const double *b, double **xp, double **h, int &nh, float *ms)
+{
+    *ms = 0.f;
+    if (c->prm.warm_order == 0) return ST_OK;
+    if (!h[0]) {
+        CKR(dalloc(c, &h[0], c->nvec));
+        CKR(dalloc(c, &h[1], c->nvec));
+    }
+    hipEvent_t a, e;
+    CK(hipEventCreate(&a));
+    CK(hipEventCreate(&e));
+    CK(hipEventRecord(a, c->s));
+    double *x1 = *xp, *x2 = h[0], *f = h[1];  // f: x3, or free
+    const int ord = std::min(nh, c->prm.warm_order);
+    bool take = false;
+    if (ord >= 1) {
+        if (ord == 2) launch_lincomb3(c->nvec, 3.0, x1, -3.0, x2, 1.0, f, c->s);  // 3 x1 - 3 x2 + x3
+        else launch_lincomb3(c->nvec, 2.0, x1, -1.0, x2, 0.0, f, c->s);            // 2 x1 - x2
+        double r_prev = 0.0, r_cand = 0.0;
+        CKR(resid_norm(c, b, x1, &r_prev));
+        CKR(resid_norm(c, b, f, &r_cand));
+        take = std::isfinite(r_cand) && r_cand < r_prev;
+    }
+    if (take) {
+        *xp = f;
+        h[0] = x1;
+        h[1] = x2;
+    } else {
+        launch_axpby<double, double>(c->nvec, 1.0, x1, 0.0, f, c->s);
+        h[0] = f;
+        h[1] = x2;
+    }
+    nh = std::min(nh + 1, 2);
+    CK(hipEventRecord(e, c->s));
+    CK(hipEventSynchronize(e));
+    hipEventElapsedTime(ms, a, e);
+    hipEventDestroy(a);
+    hipEventDestroy(e);
+    return ST_OK;
 }
 
 // ------------------------------------------------------ learned predictor
@@ -992,11 +1053,11 @@ static int predict(st_ctx *c, const double *rho_dev, double *xc)
     return ST_OK;
 }
 
-// true FP64 residual norm ||B - A x|| (scratch: the first Krylov vector)
-static int resid_norm(st_ctx *c, const double *x, double *nrm)
+// true FP64 residual norm ||b - A x|| (scratch: the first Krylov vector)
+static int resid_norm(st_ctx *c, const double *b, const double *x, double *nrm)
 {
     const Lvl &g = c->lv[0];
-    launch_residual<double>(g, c->fl[0], c->h64.al[0], c->B, x, c->V, c->part, c->s);
+    launch_residual<double>(g, c->fl[0], c->h64.al[0], b, x, c->V, c->part, c->s);
     launch_sum((int)residual_blocks(g), 1, c->part, c->red, c->s);
     double r2 = 0.0;
     CK(hipMemcpyAsync(&r2, c->red, sizeof(double), hipMemcpyDeviceToHost, c->s));
@@ -1052,10 +1113,10 @@ static int learned_guess(st_ctx *c, float *ms, int *used)
             launch_axpby<double, double>(nvc, -1.0, c->xcp, 1.0, c->xcn, c->s);  // xcn = net(rho) - net(rho_prev)
         }
         double r_base = 0.0, r_cand = 0.0;
-        CKR(resid_norm(c, c->X, &r_base));
+        CKR(resid_norm(c, c->B, c->X, &r_base));
         launch_axpby<double, double>(c->nvec, 1.0, c->X, 0.0, c->Wv, c->s);  // keep the base
         prolong_state_add(c, c->xcn, c->X);
-        CKR(resid_norm(c, c->X, &r_cand));
+        CKR(resid_norm(c, c->B, c->X, &r_cand));
         take = (std::isfinite(r_cand) && r_cand < r_base) ? 1 : 0;
         if (!take) launch_axpby<double, double>(c->nvec, 1.0, c->Wv, 0.0, c->X, c->s);
         c->learned_rejects = take ? 0 : c->learned_rejects + 1;
@@ -1075,21 +1136,26 @@ int st_solve(st_ctx *c, int init, const double *x0, double *x_out, st_stats *sta
 {
     if (!c) return ST_EINVAL;
     if (!c->rhs) return fail(c, ST_ESTATE, "st_set_rhs must precede st_solve");
+    float wms = 0.f;  // extrapolated start
     if (init == ST_INIT_HOST) {
         if (!x0) return ST_EINVAL;
         CKR(check_finite(c, x0, c->ndof, "x0"));
         CKR(stage_in(c, x0, c->ndof));
         launch_pack<double>(c->lv[0], c->abi_a, c->X, 0, c->fl[0], 1.0, c->s);
+        c->nxh = 0;  // a new sequence
     } else if (init == ST_INIT_KEEP) {
         if (!c->have_x) init = ST_INIT_ZERO;
+        else CKR(warm_extrapolate(c, c->B, &c->X, c->XH, c->nxh, &wms));
     } else if (init == ST_INIT_LEARNED) {
         float ims = 0.f;
         int used = 0;
+        if (c->have_x) CKR(warm_extrapolate(c, c->B, &c->X, c->XH, c->nxh, &wms));
+        else c->nxh = 0;
         CKR(learned_guess(c, &ims, &used));
         int rc = solve_core(c, c->B, c->X, c->XD, ST_INIT_KEEP, stats);
         if (stats) {
-            stats->init_ms = ims;
-            stats->ms += ims;
+            stats->init_ms = wms + ims;
+            stats->ms += wms + ims;
             stats->learned_used = used;
         }
         if (rc == ST_OK || rc == ST_ENOCONV) {
@@ -1105,7 +1171,12 @@ int st_solve(st_ctx *c, int init, const double *x0, double *x_out, st_stats *sta
     } else if (init != ST_INIT_ZERO) {
         return ST_EINVAL;
     }
+    if (init == ST_INIT_ZERO) c->nxh = 0;
     int rc = solve_core(c, c->B, c->X, c->XD, init == ST_INIT_HOST ? ST_INIT_KEEP : init, stats);
+    if (stats && wms > 0.f) {
+        stats->init_ms = wms;
+        stats->ms += wms;
+    }
     if (rc == ST_OK || rc == ST_ENOCONV) {
         c->have_x = true;
         remember_design(c);
@@ -1131,6 +1202,7 @@ int st_reset_solution(st_ctx *c)
 {
     if (!c) return ST_EINVAL;
     c->have_x = c->have_lam = c->have_rhop = false;
+    c->nxh = c->nlh = 0;
     return ST_OK;
 }
 
@@ -1362,14 +1434,16 @@ int st_to_step(st_ctx *c, int init, double volfrac, double move, double eta, dou
     filter_dev(c, c->rho, c->rhof, 0);
     design_to_alpha(c, c->rhof);
     int fmode = (init == ST_INIT_ZERO || !c->have_x) ? ST_INIT_ZERO : ST_INIT_KEEP;
-    float ims = 0.f;
+    float ims = 0.f, wms = 0.f;
     int used = 0;
+    if (fmode == ST_INIT_KEEP) CKR(warm_extrapolate(c, c->B, &c->X, c->XH, c->nxh, &wms));
+    else c->nxh = 0;
     if (init == ST_INIT_LEARNED && c->have_x && c->net && net_num_samples(c->net) > 0)
         CKR(learned_guess(c, &ims, &used));
     int rc = solve_core(c, c->B, c->X, c->XD, fmode, fwd);
     if (fwd) {
-        fwd->init_ms = ims;
-        fwd->ms += ims;
+        fwd->init_ms = wms + ims;
+        fwd->ms += wms + ims;
         fwd->learned_used = used;
     }
     if (rc != ST_OK) return rc;
@@ -1378,7 +1452,14 @@ int st_to_step(st_ctx *c, int init, double volfrac, double move, double eta, dou
     CKR(objective_dev(c, c->X, c->BA, J));
     // adjoint: A lam = G x (A symmetric), homogeneous Dirichlet data
     int amode = (init == ST_INIT_ZERO || !c->have_lam) ? ST_INIT_ZERO : ST_INIT_KEEP;
+    float lms = 0.f;
+    if (amode == ST_INIT_KEEP) CKR(warm_extrapolate(c, c->BA, &c->LAM, c->LH, c->nlh, &lms));
+    else c->nlh = 0;
     rc = solve_core(c, c->BA, c->LAM, c->ZERO, amode, adj);
+    if (adj) {
+        adj->init_ms = lms;
+        adj->ms += lms;
+    }
     if (rc != ST_OK) return rc;
     c->have_lam = true;
     launch_sensitivity(c->lv[0], c->rhof, alpha_params(c), c->fl[0], c->X, c->LAM, c->dj, c->s);

@@ -1,0 +1,72 @@
+"""The extrapolated start of a sequence of solves (st_params.warm_order).
+
+A solve with ST_INIT_KEEP starts from the quadratic (or linear) extrapolation of the
+sequence's previous solutions when its true FP64 residual is below that of the held
+solution.  The contract: the start is never worse than the held solution, the
+solutions and the optimisation's designs do not change beyond the solver tolerance,
+and along an optimisation run the starts are better.
+"""
+import numpy as np
+import pytest
+
+from conftest import rel
+from helpers import setup, solver_for
+from stokes_to_amd.problems import channel_problem
+
+pytestmark = pytest.mark.gpu
+
+
+def _sequence(n, k, seed=7):
+    """k designs moving smoothly from one GRF to another, then a jump to a third."""
+    _, a = setup(3, n, mean=0.3, seed=seed)
+    _, b = setup(3, n, mean=0.3, seed=seed + 1)
+    _, c = setup(3, n, mean=0.4, seed=seed + 2)
+    return [(1 - t) * a + t * b for t in np.linspace(0.0, 0.3, k)] + [c]
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_extrapolated_start_is_never_worse_than_the_held_solution(make_solver, order):
+    n = 32
+    P = channel_problem(3, n)
+    designs = _sequence(n, 7)
+    runs = {}
+    for wo in (0, order):
+        s = solver_for(make_solver, P, None, precision="fp32", rtol=1e-8, warm_order=wo)
+        out = []
+        for k, d in enumerate(designs):
+            s.set_design(d)
+            x, st = s.solve(init="zero" if k == 0 else "keep")
+            assert st["converged"] and st["relres"] <= 1e-8
+            out.append((x, st))
+        runs[wo] = out
+    for k in range(1, len(designs)):
+        (x0, s0), (x1, s1) = runs[0][k], runs[order][k]
+        # the held solutions of the two runs differ at the solver tolerance only
+        assert s1["relres0"] <= s0["relres0"] + 1e-7, (k, s1["relres0"], s0["relres0"])
+        assert rel(x1, x0) < 1e-6
+    # along the smooth part the extrapolation is taken and starts closer
+    assert sum(runs[order][k][1]["relres0"] for k in range(3, 7)) < 0.5 * sum(runs[0][k][1]["relres0"] for k in range(3, 7))
+
+
+def test_optimisation_run_with_extrapolated_starts(make_solver):
+    """A 32^3 OC run (24 iterations, forward and adjoint warm-started): the same
+    objectives and final design as with the held solution as start, fewer iterations."""
+    n, vf, its = 32, 0.25, 24
+    P = channel_problem(3, n)
+    res = {}
+    for wo in (0, 2):
+        s = solver_for(make_solver, P, np.full(n ** 3, vf), precision="fp32", rtol=1e-8, warm_order=wo)
+        J, fi, ai = [], 0, 0
+        for it in range(its):
+            o = s.to_step(vf, init="keep")
+            J.append(o["J"])
+            if it >= 3:
+                fi += o["fwd"]["iters"]
+                ai += o["adj"]["iters"]
+        res[wo] = (np.array(J), s.get_design(), fi, ai)
+    J0, d0, f0, a0 = res[0]
+    J2, d2, f2, a2 = res[2]
+    print(f"forward iterations {f0} -> {f2}, adjoint {a0} -> {a2}", flush=True)
+    assert np.max(np.abs(J2 - J0) / np.abs(J0)) < 1e-6
+    assert rel(d2, d0) < 1e-4
+    assert f2 < f0 and a2 <= a0
