@@ -206,13 +206,15 @@ __device__ __forceinline__ double residual_body(const Lvl &g, const uint8_t *fl,
     C v0 = Cvt<TS>::ld(b[n]) - s.Acell(i, j, k);
     C v1 = Cvt<TS>::ld(b[g.S + n]) - s.template Aface<0>(i, j, k);
     C v2 = Cvt<TS>::ld(b[2 * g.S + n]) - s.template Aface<1>(i, j, k);
-    r[n] = Cvt<TS>::st(v0);
-    r[g.S + n] = Cvt<TS>::st(v1);
-    r[2 * g.S + n] = Cvt<TS>::st(v2);
+    if (r) {  // without r: the norm only
+        r[n] = Cvt<TS>::st(v0);
+        r[g.S + n] = Cvt<TS>::st(v1);
+        r[2 * g.S + n] = Cvt<TS>::st(v2);
+    }
     double acc = (double)v0 * v0 + (double)v1 * v1 + (double)v2 * v2;
     if (DIM == 3) {
         C v3 = Cvt<TS>::ld(b[3 * g.S + n]) - s.template Aface<2>(i, j, k);
-        r[3 * g.S + n] = Cvt<TS>::st(v3);
+        if (r) r[3 * g.S + n] = Cvt<TS>::st(v3);
         acc += (double)v3 * v3;
     }
     return acc;

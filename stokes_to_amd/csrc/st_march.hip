@@ -117,7 +117,12 @@ __global__ void __launch_bounds__(BJ * 64) k_march(Lvl g, const uint8_t *__restr
         bf[2] = b + 2 * S;
         bf[3] = b + 3 * S;
     }
-    TY *yf[4] = {y, y + S, y + 2 * S, y + 3 * S};
+    TY *yf[4] = {y, y, y, y};  // y is null for a residual norm without the residual
+    if (y) {
+        yf[1] = y + S;
+        yf[2] = y + 2 * S;
+        yf[3] = y + 3 * S;
+    }
     auto ldx = [&](int f, I q) -> C { return (C)Cvt<TX>::ld(gld<O32>(xf[f], q)); };
     auto lda = [&](int f, I q) -> C { return (C)Cvt<TS>::ld(gld<O32>(af[f], q)); };
     auto ldb = [&](int f, I q) -> C { return (C)Cvt<TS>::ld(gld<O32>(bf[f], q)); };
@@ -308,10 +313,12 @@ __global__ void __launch_bounds__(BJ * 64) k_march(Lvl g, const uint8_t *__restr
                     v3 = bb[3] - v3;
                     acc += (double)v0 * v0 + (double)v1 * v1 + (double)v2 * v2 + (double)v3 * v3;
                 }
-                sty(0, qq, v0);
-                sty(1, qq, v1);
-                sty(2, qq, v2);
-                sty(3, qq, v3);
+                if (MODE != M_RESID || y) {  // a residual without y: its norm only
+                    sty(0, qq, v0);
+                    sty(1, qq, v1);
+                    sty(2, qq, v2);
+                    sty(3, qq, v3);
+                }
             };
             emit(cown, q, gb, pbl, pb, px, pa);
             // the padded node column k = nz and row j = ny belong to the last tiles
@@ -332,7 +339,7 @@ __global__ void __launch_bounds__(BJ * 64) k_march(Lvl g, const uint8_t *__restr
             if (lastk && lastj) emit_edge(cown + LK + 1, q + sJ + 1);
             // the row padding k in (nz, Pz) is written too (zeros), so that y covers
             // every entry of the vector: callers reuse buffers across precisions
-            if (k0 + BK == g.nz && g.nz + 1 + tk < g.Pz) {
+            if (y && k0 + BK == g.nz && g.nz + 1 + tk < g.Pz) {
                 const I qp = (I)i * sI + (I)j * sJ + (I)(g.nz + 1 + tk);
 #pragma unroll
                 for (int f = 0; f < 4; f++) {

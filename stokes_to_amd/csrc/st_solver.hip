@@ -1053,11 +1053,11 @@ static int predict(st_ctx *c, const double *rho_dev, double *xc)
     return ST_OK;
 }
 
-// true FP64 residual norm ||b - A x|| (scratch: the first Krylov vector)
+// true FP64 residual norm ||b - A x|| (the residual itself is not stored)
 static int resid_norm(st_ctx *c, const double *b, const double *x, double *nrm)
 {
     const Lvl &g = c->lv[0];
-    launch_residual<double>(g, c->fl[0], c->h64.al[0], b, x, c->V, c->part, c->s);
+    launch_residual<double>(g, c->fl[0], c->h64.al[0], b, x, (double *)nullptr, c->part, c->s);
     launch_sum((int)residual_blocks(g), 1, c->part, c->red, c->s);
     double r2 = 0.0;
     CK(hipMemcpyAsync(&r2, c->red, sizeof(double), hipMemcpyDeviceToHost, c->s));
