@@ -86,6 +86,10 @@ struct st_ctx {
     // before that; nxh / nlh of them valid
     double *XH[2] = {nullptr, nullptr}, *LH[2] = {nullptr, nullptr};
     int nxh = 0, nlh = 0;
+    // the FP32 path's first residual of the next solve, already in u_0 (set by the
+    // extrapolated start when it computed it; consumed or dropped by the next solve)
+    bool r0_ready = false;
+    double r0_beta = 0.0;
     // kernel timing: event pairs around finest-level launches
     bool prof = false;
     std::vector<hipEvent_t> evpool[ST_PROF_NKIND];
@@ -466,7 +470,7 @@ static void inner_apply(st_ctx *c, const float *z, double *w) { apply64(c, z, w)
 // FP32 operator the Arnoldi estimate of the residual is only good to ~1e-6 of the
 // cycle's starting residual, so a cycle also ends there and the FP64 residual of the
 // restart takes over.
-template <typename TV, typename TW> static int fgmres(st_ctx *c, const double *b, double *x, st_stats *st)
+template <typename TV, typename TW> static int fgmres(st_ctx *c, const double *b, double *x, st_stats *st, bool use_r0)
 {
     // LAZY (Krylov vectors and operator in one precision): the basis is stored
     // unnormalised, v_i = sig_i u_i, and A M u_j is written straight into the slot of
@@ -497,13 +501,18 @@ template <typename TV, typename TW> static int fgmres(st_ctx *c, const double *b
         double *r0 = std::is_same<TV, double>::value ? c->V : c->Wv;  // FP64 residual
         // FP32 path on a tiled level: the FP64 residual goes straight into u_0 in FP32
         const bool r_f32 = LAZY && std::is_same<TV, float>::value && march_ok(g);
-        if (r_f32) march_residual_f32(g, c->fl[0], c->h64.al[0], b, x, reinterpret_cast<float *>(Vb), c->part, c->s);
-        else launch_residual<double>(g, c->fl[0], c->h64.al[0], b, x, r0, c->part, c->s);
-        launch_sum(rb, 1, c->part, c->red, c->s);
-        double r2 = 0.0;
-        CK(hipMemcpyAsync(&r2, c->red, sizeof(double), hipMemcpyDeviceToHost, c->s));
-        CK(hipStreamSynchronize(c->s));
-        double beta = std::sqrt(r2);
+        double beta;
+        if (r_f32 && use_r0 && it == 0) {  // the start's residual is already in u_0
+            beta = c->r0_beta;
+        } else {
+            if (r_f32) march_residual_f32(g, c->fl[0], c->h64.al[0], b, x, reinterpret_cast<float *>(Vb), c->part, c->s);
+            else launch_residual<double>(g, c->fl[0], c->h64.al[0], b, x, r0, c->part, c->s);
+            launch_sum(rb, 1, c->part, c->red, c->s);
+            double r2 = 0.0;
+            CK(hipMemcpyAsync(&r2, c->red, sizeof(double), hipMemcpyDeviceToHost, c->s));
+            CK(hipStreamSynchronize(c->s));
+            beta = std::sqrt(r2);
+        }
         if (!std::isfinite(beta)) return fail(c, ST_ENAN, "non-finite residual");
         rr = beta / bn;
         if (rr0 < 0) rr0 = rr;
@@ -943,10 +952,12 @@ static int solve_core(st_ctx *c, const double *b, double *x, const double *xd, i
     CK(hipEventRecord(c->e0, c->s));
     if (c->dirty) CKR(setup_hierarchy(c));
     local.setup_ms = c->last_setup_ms;
+    const bool use_r0 = c->r0_ready && init == ST_INIT_KEEP;
+    c->r0_ready = false;
     if (init == ST_INIT_ZERO) launch_axpby<double, double>(c->nvec, 1.0, xd, 0.0, x, c->s);
-    int rc = (c->prm.precision == ST_FP64)   ? fgmres<double, double>(c, b, x, &local)
-             : (c->prm.precision == ST_FP32) ? fgmres<float, float>(c, b, x, &local)
-                                              : fgmres<float, double>(c, b, x, &local);
+    int rc = (c->prm.precision == ST_FP64)   ? fgmres<double, double>(c, b, x, &local, false)
+             : (c->prm.precision == ST_FP32) ? fgmres<float, float>(c, b, x, &local, use_r0)
+                                              : fgmres<float, double>(c, b, x, &local, false);
     CK(hipEventRecord(c->e1, c->s));
     CK(hipEventSynchronize(c->e1));
     float ms = 0.f;
@@ -967,10 +978,12 @@ static int solve_core(st_ctx *c, const double *b, double *x, const double *xd, i
 // call *xp is the start and h[0] = x1, h[1] = x2: the three buffers are rotated; x1 is
 // copied only when the candidate loses (or when there is nothing to extrapolate yet).
 static int resid_norm(st_ctx *c, const double *b, const double *x, double *nrm);
+static int resid_norm_u0(st_ctx *c, const double *b, const double *x, double *nrm);
 
 static int warm_extrapolate(st_ctx *c, const double *b, double **xp, double **h, int &nh, float *ms)
 {
     *ms = 0.f;
+    c->r0_ready = false;
     if (c->prm.warm_order == 0) return ST_OK;
     if (!h[0]) {
         CKR(dalloc(c, &h[0], c->nvec));
@@ -988,8 +1001,14 @@ static int warm_extrapolate(st_ctx *c, const double *b, double **xp, double **h,
         else launch_lincomb3(c->nvec, 2.0, x1, -1.0, x2, 0.0, f, c->s);            // 2 x1 - x2
         double r_prev = 0.0, r_cand = 0.0;
         CKR(resid_norm(c, b, x1, &r_prev));
-        CKR(resid_norm(c, b, f, &r_cand));
+        // on the FP32 path the candidate's residual goes where the solve's first
+        // residual goes (u_0, in FP32), so the solve need not compute it again
+        const bool f32 = c->prm.precision == ST_FP32 && march_ok(c->lv[0]);
+        if (f32) CKR(resid_norm_u0(c, b, f, &r_cand));
+        else CKR(resid_norm(c, b, f, &r_cand));
         take = std::isfinite(r_cand) && r_cand < r_prev;
+        c->r0_ready = take && f32;
+        c->r0_beta = r_cand;
     }
     if (take) {
         *xp = f;
@@ -1050,6 +1069,20 @@ static int predict(st_ctx *c, const double *rho_dev, double *xc)
     restrict_state(c, c->XD, c->xdc);
     if (net_predict(c->net, c->lv[c->lc], c->n, c->n / c->lv[c->lc].nx, rho_dev, c->fl[c->lc], c->xdc, xc) != 0)
         return fail(c, ST_EINVAL, "predictor/level mismatch");
+    return ST_OK;
+}
+
+// the same with the residual stored in FP32 as the first Krylov vector u_0 (the
+// FP32 path on a tiled finest level)
+static int resid_norm_u0(st_ctx *c, const double *b, const double *x, double *nrm)
+{
+    const Lvl &g = c->lv[0];
+    march_residual_f32(g, c->fl[0], c->h64.al[0], b, x, reinterpret_cast<float *>(c->V), c->part, c->s);
+    launch_sum((int)residual_blocks(g), 1, c->part, c->red, c->s);
+    double r2 = 0.0;
+    CK(hipMemcpyAsync(&r2, c->red, sizeof(double), hipMemcpyDeviceToHost, c->s));
+    CK(hipStreamSynchronize(c->s));
+    *nrm = std::sqrt(r2);
     return ST_OK;
 }
 
@@ -1118,6 +1151,7 @@ static int learned_guess(st_ctx *c, float *ms, int *used)
         prolong_state_add(c, c->xcn, c->X);
         CKR(resid_norm(c, c->B, c->X, &r_cand));
         take = (std::isfinite(r_cand) && r_cand < r_base) ? 1 : 0;
+        if (take) c->r0_ready = false;  // the start changed
         if (!take) launch_axpby<double, double>(c->nvec, 1.0, c->Wv, 0.0, c->X, c->s);
         c->learned_rejects = take ? 0 : c->learned_rejects + 1;
     }
