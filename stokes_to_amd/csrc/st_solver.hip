@@ -944,7 +944,21 @@ int st_set_rhs(st_ctx *c, const double *body, const double *dirichlet)
     return ST_OK;
 }
 
+static int solve_core_(st_ctx *c, const double *b, double *x, const double *xd, int init, st_stats *st);
+
+// A failed solve (other than the iteration cap) leaves x undefined: the sequence of
+// held solutions it belonged to starts again
 static int solve_core(st_ctx *c, const double *b, double *x, const double *xd, int init, st_stats *st)
+{
+    const int rc = solve_core_(c, b, x, xd, init, st);
+    if (rc != ST_OK && rc != ST_ENOCONV) {
+        if (x == c->X) c->nxh = 0;
+        if (x == c->LAM) c->nlh = 0;
+    }
+    return rc;
+}
+
+static int solve_core_(st_ctx *c, const double *b, double *x, const double *xd, int init, st_stats *st)
 {
     if (!c->geom || !c->alpha) return fail(c, ST_ESTATE, "geometry and penalty/design must be set before solving");
     st_stats local;

@@ -145,3 +145,29 @@ def test_fp32_solve_is_unchanged_by_fp64_work_on_shared_buffers(make_solver):
     x2, st2 = s.solve()
     assert st1["converged"] and st2["iters"] == st1["iters"]
     assert np.array_equal(x1, x2)
+
+
+def test_failed_solve_in_a_warm_started_sequence(make_solver):
+    """A sequence of warm-started (extrapolated) solves, a solve that fails on a NaN
+    design in the middle, then the sequence goes on: the next solve converges to the
+    solution of a fresh context."""
+    P, rho = setup(3, 16)
+    s = solver_for(make_solver, P, rho, rtol=1e-10, precision="fp32")
+    s.solve()
+    for f in (0.95, 0.9, 0.85):
+        s.set_design(rho * f)
+        _, st = s.solve(init="keep")
+        assert st["converged"]
+    bad = rho.copy()
+    bad[3] = np.nan
+    d = DeviceArray(bad)
+    assert s.lib.st_set_design_device(s.h, ctypes.cast(d.ptr, ctypes.POINTER(ctypes.c_double))) == 0
+    d.free()
+    with pytest.raises(StokesError) as e:
+        s.solve(init="keep")
+    assert e.value.code == ST_ENAN
+    s.set_design(rho * 0.8)
+    x, st = s.solve(init="keep")
+    ref = solver_for(make_solver, P, rho * 0.8, rtol=1e-10, precision="fp32")
+    xr, _ = ref.solve()
+    assert st["converged"] and rel(x, xr) < 1e-7
