@@ -74,9 +74,16 @@ void launch_energy(const Lvl &, const uint8_t *fl, const double *al, const doubl
 unsigned energy_blocks(const Lvl &);
 void launch_filter(const Lvl &, double R, const double *W, const double *in, double *out, int transpose, hipStream_t);
 void launch_filter_w(const Lvl &, double R, double *W, hipStream_t);
-unsigned oc_blocks(long n);
-void launch_oc_step(long n, const double *rho, const double *dc, const double *dv, double lm, double move, double eta,
-                    double rho_min, double *rn, double *part, hipStream_t);
+constexpr int OC_K = 16;  // multipliers per pass of the OC search
+struct OcFactors {
+    double f[OC_K];
+};
+unsigned oc_grid(long n);
+void launch_oc_base(long n, const double *dc, const double *dv, double eta, double *base, hipStream_t);
+void launch_oc_vols(long n, const double *rho, const double *base, const double *dv, const OcFactors &f, double move,
+                    double rho_min, double *part, hipStream_t);
+void launch_oc_apply(long n, const double *rho, const double *base, const double *dv, double fl, double move,
+                     double rho_min, double *rn, double *part, hipStream_t);
 unsigned absmax_blocks(long n);
 void launch_absdiff_max(long n, const double *a, const double *b, double *part, hipStream_t);
 void launch_sum(int np, int nvec, const double *part, double *out, hipStream_t);

@@ -1047,26 +1047,55 @@ __global__ void __launch_bounds__(256) k_filter(int dim, int nx, int ny, int nz,
     out[t] = transpose ? s : s / W[t];
 }
 
-// one OC bisection step: rho_new for multiplier lm and partial sums of rho_new.dv
-__global__ void __launch_bounds__(256) k_oc_step(long n, const double *rho, const double *dc, const double *dv,
-                                                 double lm, double move, double eta, double rho_min, double *rn,
-                                                 double *part)
+// OC in factored form: at multiplier l the update is rho * base * l^-eta clamped to the
+// move limits and bounds, base = (-dc/dv)^eta where -dc/dv > 0 (else 0), so the search
+// over l needs no pow per cell and step.  k_oc_vols evaluates the volume sum(v dv) for
+// OC_K multipliers in one pass (factors f_k = l_k^-eta); k_oc_apply writes the update
+// for one and its volume.
+__global__ void __launch_bounds__(256) k_oc_base(long n, const double *dc, const double *dv, double eta, double *base)
+{
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+        const double b0 = -dc[t] / dv[t];
+        base[t] = (b0 > 0.0) ? pow(b0, eta) : 0.0;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_oc_vols(long n, const double *rho, const double *base, const double *dv,
+                                                 OcFactors f, double move, double rho_min, double *part)
 {
     __shared__ double sh[16];
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    double acc = 0.0;
-    if (t < n) {
-        double be = -dc[t] / (dv[t] * lm);
-        if (be < 0) be = 0;
-        double v = rho[t] * pow(be, eta);
-        double lo = rho[t] - move, hi = rho[t] + move;
-        if (lo < rho_min) lo = rho_min;
-        if (hi > 1.0) hi = 1.0;
-        v = v < lo ? lo : (v > hi ? hi : v);
-        rn[t] = v;
-        acc = v * dv[t];
+    double acc[OC_K];
+#pragma unroll
+    for (int k = 0; k < OC_K; k++) acc[k] = 0.0;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+        const double r = rho[t], rb = r * base[t], w = dv[t];
+        const double lo = fmax(r - move, rho_min), hi = fmin(r + move, 1.0);
+#pragma unroll
+        for (int k = 0; k < OC_K; k++) {
+            const double v = rb * f.f[k];
+            acc[k] += (v < lo ? lo : (v > hi ? hi : v)) * w;
+        }
     }
-    double s = block_sum(acc, sh);
+    for (int k = 0; k < OC_K; k++) {
+        const double s = block_sum(acc[k], sh);
+        if (threadIdx.x == 0) part[(long)k * gridDim.x + blockIdx.x] = s;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_oc_apply(long n, const double *rho, const double *base, const double *dv,
+                                                  double fl, double move, double rho_min, double *rn, double *part)
+{
+    __shared__ double sh[16];
+    double acc = 0.0;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+        const double r = rho[t];
+        const double lo = fmax(r - move, rho_min), hi = fmin(r + move, 1.0);
+        const double v0 = r * base[t] * fl;
+        const double v = v0 < lo ? lo : (v0 > hi ? hi : v0);
+        rn[t] = v;
+        acc += v * dv[t];
+    }
+    const double s = block_sum(acc, sh);
     if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
@@ -1425,11 +1454,20 @@ void launch_filter_w(const Lvl &g, double R, double *W, hipStream_t s)
     if (r < 0) r = 0;
     hipLaunchKernelGGL(k_filter_w, dim3(nb(nc)), dim3(256), 0, s, g.dim, g.nx, g.ny, g.nz, r, R, W);
 }
-unsigned oc_blocks(long n) { return nb(n); }
-void launch_oc_step(long n, const double *rho, const double *dc, const double *dv, double lm, double move, double eta,
-                    double rho_min, double *rn, double *part, hipStream_t s)
+unsigned oc_grid(long n) { return nb_grid(n); }
+void launch_oc_base(long n, const double *dc, const double *dv, double eta, double *base, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_oc_step, dim3(nb(n)), dim3(256), 0, s, n, rho, dc, dv, lm, move, eta, rho_min, rn, part);
+    hipLaunchKernelGGL(k_oc_base, dim3(nb_grid(n)), dim3(256), 0, s, n, dc, dv, eta, base);
+}
+void launch_oc_vols(long n, const double *rho, const double *base, const double *dv, const OcFactors &f, double move,
+                    double rho_min, double *part, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_oc_vols, dim3(nb_grid(n)), dim3(256), 0, s, n, rho, base, dv, f, move, rho_min, part);
+}
+void launch_oc_apply(long n, const double *rho, const double *base, const double *dv, double fl, double move,
+                     double rho_min, double *rn, double *part, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_oc_apply, dim3(nb_grid(n)), dim3(256), 0, s, n, rho, base, dv, fl, move, rho_min, rn, part);
 }
 unsigned absmax_blocks(long n) { return nb_grid(n); }
 void launch_absdiff_max(long n, const double *a, const double *b, double *part, hipStream_t s)

@@ -1356,19 +1356,34 @@ static int oc_dev(st_ctx *c, const double *rho, const double *dc, double volfrac
     const long n = c->ncell;
     launch_fill<double>(n, c->tmpc, 1.0 / (double)n, c->s);
     filter_dev(c, c->tmpc, c->dv, 1);  // dV/drho = H^T(1/(n W))
-    const int nb = (int)oc_blocks(n);
+    // search for the multiplier: the volume falls with it; each pass evaluates OC_K
+    // multipliers spread over the bracket and keeps the sub-bracket where the volume
+    // crosses volfrac (17-fold narrowing per pass, the bisection's tolerance)
+    double *base = c->tmpc;  // free once dv is formed
+    launch_oc_base(n, dc, c->dv, eta, base, c->s);
+    const int gb = (int)oc_grid(n);
     double l1 = 0.0, l2 = 1e9, vol = 0.0;
-    int it = 0;
-    while ((l2 - l1) > 1e-13 * (l1 + l2) && it < 200) {
-        double lm = 0.5 * (l1 + l2);
-        launch_oc_step(n, rho, dc, c->dv, lm, move, eta, rho_min, rn, c->part, c->s);
-        launch_sum(nb, 1, c->part, c->red, c->s);
-        CK(hipMemcpyAsync(&vol, c->red, sizeof(double), hipMemcpyDeviceToHost, c->s));
+    for (int pass = 0; (l2 - l1) > 1e-13 * (l1 + l2) && pass < 60; pass++) {
+        OcFactors f;
+        double lk[OC_K], vk[OC_K];
+        for (int k = 0; k < OC_K; k++) {
+            lk[k] = l1 + (l2 - l1) * (double)(k + 1) / (double)(OC_K + 1);
+            f.f[k] = std::pow(lk[k], -eta);
+        }
+        launch_oc_vols(n, rho, base, c->dv, f, move, rho_min, c->part, c->s);
+        launch_sum(gb, OC_K, c->part, c->red, c->s);
+        CK(hipMemcpyAsync(vk, c->red, sizeof(double) * OC_K, hipMemcpyDeviceToHost, c->s));
         CK(hipStreamSynchronize(c->s));
-        if (vol > volfrac) l1 = lm;
-        else l2 = lm;
-        it++;
+        int k = 0;
+        while (k < OC_K && vk[k] > volfrac) k++;
+        const double n1 = (k == 0) ? l1 : lk[k - 1], n2 = (k == OC_K) ? l2 : lk[k];
+        l1 = n1;
+        l2 = n2;
     }
+    launch_oc_apply(n, rho, base, c->dv, std::pow(0.5 * (l1 + l2), -eta), move, rho_min, rn, c->part, c->s);
+    launch_sum(gb, 1, c->part, c->red, c->s);
+    CK(hipMemcpyAsync(&vol, c->red, sizeof(double), hipMemcpyDeviceToHost, c->s));
+    CK(hipStreamSynchronize(c->s));
     launch_absdiff_max(n, rn, rho, c->part, c->s);
     std::vector<double> pm(absmax_blocks(n));
     CK(hipMemcpyAsync(pm.data(), c->part, sizeof(double) * pm.size(), hipMemcpyDeviceToHost, c->s));
