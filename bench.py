@@ -57,7 +57,8 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=65, help="timed design iterations (solves)")
     ap.add_argument("--warmup", type=int, default=15, help="untimed iterations (online training window)")
-    ap.add_argument("--workload", default="to", choices=["to", "grf", "net"])
+    ap.add_argument("--workload", default="to", choices=["to", "grf", "net", "to-warm"])
+    ap.add_argument("--to-iters", type=int, default=30, help="design iterations of the to-warm workload")
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--volfrac", type=float, default=0.2)
     ap.add_argument("--mean", type=float, default=0.2, help="GRF mean (grf workload)")
@@ -257,6 +258,26 @@ def run_to(args, prob):
     return {nm: summary(v) for nm, v in st.items()}, extra
 
 
+def run_to_warm(args, prob):
+    """One TO run with warm starts only (FP32/BF16 multigrid, extrapolated starts, no
+    network) for --to-iters design iterations: the workload for kernel traces of whole
+    design steps (forward solve, objective, adjoint solve, sensitivity, filter, OC)."""
+    s = new_ctx(args, prob, args.mp)
+    s.set_design(np.full(args.n ** 3, args.volfrac))
+    rows = []
+    for it in range(args.to_iters):
+        t0 = time.perf_counter()
+        o = s.to_step(args.volfrac, init="keep")
+        step_ms = (time.perf_counter() - t0) * 1e3
+        f, a = o["fwd"], o["adj"]
+        rows.append({"it": it, "step_ms": round(step_ms, 2), "fwd_iters": int(f["iters"]), "fwd_ms": round(f["ms"], 2),
+                     "adj_iters": int(a["iters"]), "adj_ms": round(a["ms"], 2), "J": o["J"]})
+        log(args, f"[bench] it {it:2d} step {step_ms:.1f} ms | fwd {f['iters']} it {f['ms']:.1f} ms | "
+                  f"adj {a['iters']} it {a['ms']:.1f} ms | J {o['J']:.9g}")
+    s.close()
+    return rows
+
+
 def run_grf(args, prob):
     import torch
     from stokes_to_amd.problems import grf_density
@@ -453,6 +474,17 @@ def main():
         log(args, f"[bench] plain {plain['median_ms']:.1f} ms, keep+{args.mp} {keep['median_ms']:.1f} ms, "
                   f"learned+{args.mp} {learned['median_ms']:.1f} ms (median of {len(timed)}); final designs "
                   f"{extra['final_design_rel_l2_learned_vs_plain']:.2e} (learned vs plain)")
+    elif args.workload == "to-warm":
+        rows = run_to_warm(args, prob)
+        late = rows[len(rows) // 2:]
+        value = float(np.median([r["step_ms"] for r in late]))
+        out.update({"metric": "ms per warm-started design step (forward + adjoint solve, objective, sensitivity, "
+                              "filter, OC)", "value": round(value, 3), "ms_per_step": round(value, 3),
+                    "dtype": f"f64+{args.mp}", "config": {"workload": f"3D {args.n}^3 TO, {args.to_iters} OC "
+                                                                    f"iterations, warm starts", "n": args.n},
+                    "iterations": rows, "roofline": None, "cpu_baseline": None})
+        print(json.dumps(out), flush=True)
+        return 0
     elif args.workload == "net":
         r = run_net(args, prob)
         value = r["train_ms"] / max(args.epochs, 1) / 2
