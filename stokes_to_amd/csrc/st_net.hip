@@ -149,6 +149,126 @@ __global__ void __launch_bounds__(64) k_conv_wgrad(int R, const bf16 *__restrict
     for (int q = 0; q < 4; q++) o[(long)q * Cout] = acc[q];
 }
 
+// Weight gradient from channel-first staged operands (R % 4 == 0).  The MFMA wants
+// four consecutive voxels of one channel per lane (K = voxels); the activations and
+// gradients are channel-last, so both are staged once per layer:
+//   xs[cb][s][ci][(i+1) P + (j+1)][k] = x(i, j, k + s - 1) of channel cb*16 + ci, zero
+//     outside the grid (P = R + 2: a zero row and plane all round; s: the three k shifts)
+//   gs[co][v] = gz(v, co)
+// so every operand quad is one aligned 8-byte load.  One wave accumulates all taps of
+// one (ci block, co block) over its voxel groups and writes its partial.
+// one (i+1, j+1) row of xs for one ci block per workgroup (grid: P*P rows x ncib): the
+// row's R voxels x 16 channels are read channel-contiguous into LDS with a zero at
+// each end, then written as the three k-shifted rows of every channel; pad rows are zero
+__global__ void __launch_bounds__(256) k_wg_stage_x(int R, const bf16 *__restrict__ x, int csx, int cix, int ncib,
+                                                    bf16 *__restrict__ xs)
+{
+    __shared__ uint16_t sm[16][130];  // R <= 128
+    const int P = R + 2;
+    const int row = blockIdx.x, cb = blockIdx.y;
+    const int i = row / P - 1, j = row % P - 1;
+    const bool pad = i < 0 || i >= R || j < 0 || j >= R;
+    if (!pad) {
+        for (int t = threadIdx.x; t < 16 * R; t += blockDim.x) {
+            const int ci = t & 15, k = t >> 4;
+            sm[ci][k + 1] = x[(((long)i * R + j) * R + k) * csx + cix + cb * 16 + ci].v;
+        }
+        if (threadIdx.x < 16) {
+            sm[threadIdx.x][0] = 0;
+            sm[threadIdx.x][R + 1] = 0;
+        }
+    }
+    __syncthreads();
+    const long plane = (long)P * P * R;
+    bf16 *o = xs + (long)cb * 3 * 16 * plane + (long)row * R;
+    for (int t = threadIdx.x; t < 3 * 16 * R; t += blockDim.x) {
+        const int k = t % R, ci = (t / R) % 16, sh = t / (16 * R);
+        bf16 v;
+        v.v = pad ? (uint16_t)0 : sm[ci][k + sh];
+        o[((long)sh * 16 + ci) * plane + k] = v;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_wg_stage_g(long nv, int C, const bf16 *__restrict__ gz, bf16 *__restrict__ gs)
+{
+    const long tot = nv * C;
+    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < tot; q += (long)gridDim.x * blockDim.x) {
+        const int co = (int)(q / nv);
+        const long v = q % nv;
+        gs[q] = gz[v * C + co];
+    }
+}
+
+// grid (chunks, ci blocks x co blocks), 4 waves per block, gpw voxel groups of 16 per
+// wave; part[split][t][ci][co], split = chunk * 4 + wave (every entry of every split
+// is written, zeros for a wave past the last group)
+template <int NTAP>
+__global__ void __launch_bounds__(256) k_wg_staged(int R, const bf16 *__restrict__ xs, const bf16 *__restrict__ gs,
+                                                   int ncib, int ncob, long gpw, float *__restrict__ part)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int cb = blockIdx.y / ncob, ob = blockIdx.y % ncob;
+    const long nv = (long)R * R * R, ngroups = nv / 16;
+    const int P = R + 2;
+    const int m = lane & 15, qd = lane >> 4;
+    const bf16 *xb = xs + (long)cb * 3 * 16 * P * P * R;
+    const bf16 *gb = gs + (long)(ob * 16 + m) * nv;
+    f4 acc[NTAP];
+#pragma unroll
+    for (int t = 0; t < NTAP; t++) acc[t] = f4{0.f, 0.f, 0.f, 0.f};
+    const long split = (long)blockIdx.x * 4 + w;
+    const long g0 = split * gpw;
+    const long g1 = (g0 + gpw < ngroups) ? g0 + gpw : ngroups;
+    for (long g = g0; g < g1; g++) {
+        const long v = g * 16 + 4 * qd;  // this lane's quad of voxels (one k row: R % 4 == 0)
+        const int k = (int)(v % R), j = (int)((v / R) % R), i = (int)(v / ((long)R * R));
+        const bf4 bq = *(const bf4 *)(gb + v);  // B[k = voxel][n = co]
+#pragma unroll
+        for (int t = 0; t < NTAP; t++) {
+            int a, b, c;
+            tap_off(t, NTAP, a, b, c);
+            const bf4 aq = *(const bf4 *)(xb + ((((long)(c + 1) * 16 + m) * P + (i + a + 1)) * P + (j + b + 1)) * R + k);
+            acc[t] = mfma16(as_s4(aq), as_s4(bq), acc[t]);  // A[m = ci][k = voxel]
+        }
+    }
+    const int Cin = ncib * 16, Cout = ncob * 16;
+#pragma unroll
+    for (int t = 0; t < NTAP; t++)
+        for (int q = 0; q < 4; q++)
+            part[((split * NTAP + t) * Cin + cb * 16 + 4 * qd + q) * Cout + ob * 16 + m] = acc[t][q];
+}
+
+// first stage of the reduction of many partials: group g of the nsplit partials of
+// every entry summed into part2[g][q] (grid: entry blocks x groups)
+__global__ void __launch_bounds__(256) k_reduce_parts_grouped(long n, int nsplit, const float *part, float *part2)
+{
+    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const int G = gridDim.y, g = blockIdx.y;
+    const int p0 = (int)((long)nsplit * g / G), p1 = (int)((long)nsplit * (g + 1) / G);
+    float s = 0.f;
+    for (int p = p0; p < p1; p++) s += part[(long)p * n + q];
+    part2[(long)g * n + q] = s;
+}
+
+// bias gradient partials from the staged gradient gs[co][v]: block (g, co) sums its
+// segment of the row; part[g][co]
+__global__ void __launch_bounds__(256) k_rowsum_parts(long nv, const bf16 *__restrict__ gs, float *part)
+{
+    __shared__ float sh[256];
+    const int G = gridDim.x, g = blockIdx.x, co = blockIdx.y, C = gridDim.y;
+    const long v0 = nv * g / G, v1 = nv * (g + 1) / G;
+    float s = 0.f;
+    for (long v = v0 + threadIdx.x; v < v1; v += blockDim.x) s += to_f(gs[(long)co * nv + v]);
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = blockDim.x >> 1; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[(long)g * C + co] = sh[0];
+}
+
 __global__ void k_reduce_parts(long n, int nsplit, const float *part, float *out)
 {
     long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -456,6 +576,50 @@ namespace st {
 
 namespace {
 inline unsigned nblk(long n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
+inline unsigned nblk_cap(long n) { const unsigned b = nblk(n); return b > 8192 ? 8192 : (b < 1 ? 1 : b); }
+
+// staged weight gradient (R % 4 == 0): about 2048 waves per layer, each over gpw groups
+// of 16 voxels of one (ci block, co block)
+struct WgPlan {
+    long nchunks, gpw, nsplit;
+};
+WgPlan wg_plan(int R, int cin, int cout)
+{
+    const long ngroups = (long)R * R * R / 16, npairs = (long)(cin / 16) * (cout / 16);
+    long nch = (512 + npairs - 1) / npairs;
+    long gpw = (ngroups + 4 * nch - 1) / (4 * nch);
+    if (gpw < 1) gpw = 1;
+    nch = (ngroups + 4 * gpw - 1) / (4 * gpw);
+    return {nch, gpw, nch * 4};
+}
+long wg_xs_elems(int R, int cin)
+{
+    const long P = R + 2;
+    return (long)(cin / 16) * 3 * 16 * P * P * R;
+}
+bool wg_staged_ok(int R) { return R >= 4 && R % 4 == 0 && R <= 128; }
+constexpr int WG_G = 32;  // groups of the first reduction stage (weights and bias)
+// gW = sum over voxels of x(v + o_t) gz(v) for every tap, gb = sum over voxels of gz(v)
+// (xs, gs: staging buffers of wg_xs_elems(R, cin) and R^3 cout elements; part:
+// wg_plan(...).nsplit * ntap cin cout; part2: WG_G * max(ntap cin cout, cout))
+void wgrad_staged(int R, int ntap, int cin, int cout, const bf16 *x, int csx, int cix, const bf16 *gz, bf16 *xs,
+                  bf16 *gs, float *part, float *part2, float *gW, float *gb, hipStream_t s)
+{
+    const WgPlan pl = wg_plan(R, cin, cout);
+    const long nv = (long)R * R * R, nw = (long)ntap * cin * cout;
+    hipLaunchKernelGGL(k_wg_stage_x, dim3((unsigned)((R + 2) * (R + 2)), (unsigned)(cin / 16)), dim3(256), 0, s, R, x, csx,
+                       cix, cin / 16, xs);
+    hipLaunchKernelGGL(k_wg_stage_g, dim3(nblk_cap(nv * cout)), dim3(256), 0, s, nv, cout, gz, gs);
+    const dim3 grid((unsigned)pl.nchunks, (unsigned)((cin / 16) * (cout / 16)));
+    if (ntap == 27)
+        hipLaunchKernelGGL((k_wg_staged<27>), grid, dim3(256), 0, s, R, xs, gs, cin / 16, cout / 16, pl.gpw, part);
+    else
+        hipLaunchKernelGGL((k_wg_staged<1>), grid, dim3(256), 0, s, R, xs, gs, cin / 16, cout / 16, pl.gpw, part);
+    hipLaunchKernelGGL(k_reduce_parts_grouped, dim3(nblk(nw), WG_G), dim3(256), 0, s, nw, (int)pl.nsplit, part, part2);
+    hipLaunchKernelGGL(k_reduce_parts, dim3(nblk(nw)), dim3(256), 0, s, nw, WG_G, part2, gW);
+    hipLaunchKernelGGL(k_rowsum_parts, dim3(WG_G, cout), dim3(256), 0, s, nv, gs, part2);
+    hipLaunchKernelGGL(k_reduce_parts, dim3(nblk(cout)), dim3(256), 0, s, (long)cout, WG_G, part2, gb);
+}
 
 struct Layer {
     NetLayerSpec sp;
@@ -475,6 +639,8 @@ struct NetImpl {
     bf16 *X0, *E1a, *D1in, *P1, *E2a, *D2in, *P2, *E3a, *E3b, *D2a, *D2b, *D1a, *D1b, *gz;
     float *OUT, *gOUT, *gD1b, *gD1a, *gD1in, *gD2b, *gD2a, *gD2in, *gE3b, *gE3a, *gP2, *gE2a, *gP1, *gE1a;
     float *part, *feat, *tgt_raw, *tgt, *scale;
+    bf16 *wxs = nullptr, *wgs = nullptr;  // staging of the weight gradient
+    float *part2 = nullptr;               // its first reduction stage
     double *lpart;
     bool scaled = false;
     float hscale[NET_NOUT] = {1, 1, 1, 1};
@@ -523,6 +689,10 @@ struct NetImpl {
         const NetLayerSpec &p = L[l].sp;
         const int Rl = res(p.lvl);
         const long n = nv(p.lvl);
+        if (wg_staged_ok(Rl)) {
+            wgrad_staged(Rl, p.ntap, p.cin, p.cout, x, csx, cix, gzin, wxs, wgs, part, part2, L[l].gW, L[l].gb, s);
+            return;
+        }
         int ns = nsplit;
         long chunk = (n + ns - 1) / ns;
         chunk = (chunk + 15) / 16 * 16;
@@ -652,7 +822,7 @@ NetImpl *net_create(int R, int max_samples, unsigned seed, hipStream_t s)
     n->maxs = max_samples;
     n->s = s;
     bool ok = true;
-    long maxparts = 0;
+    long maxparts = 0, wgparts = 0, wgx = 0, wgg = 0;
     for (int l = 0; l < NET_NLAYER && ok; l++) {
         Layer &L = n->L[l];
         L.sp = NET_SPEC[l];
@@ -662,6 +832,12 @@ NetImpl *net_create(int R, int max_samples, unsigned seed, hipStream_t s)
              n->alloc(&L.mb, L.sp.cout) && n->alloc(&L.vb, L.sp.cout) && n->alloc(&L.gW, L.nw) &&
              n->alloc(&L.gb, L.sp.cout) && n->alloc(&L.Wp, np) && n->alloc(&L.Wq, np);
         if (L.nw > maxparts) maxparts = L.nw;
+        const int Rl = R >> L.sp.lvl;
+        if (wg_staged_ok(Rl)) {
+            wgparts = std::max(wgparts, wg_plan(Rl, L.sp.cin, L.sp.cout).nsplit * L.nw);
+            wgx = std::max(wgx, wg_xs_elems(Rl, L.sp.cin));
+            wgg = std::max(wgg, (long)Rl * Rl * Rl * L.sp.cout);
+        }
     }
     const long v0 = n->nv0, v1 = n->nv1, v2 = n->nv2;
     ok = ok && n->alloc(&n->X0, v0 * 16) && n->alloc(&n->E1a, v0 * 16) && n->alloc(&n->D1in, v0 * 48) &&
@@ -674,7 +850,8 @@ NetImpl *net_create(int R, int max_samples, unsigned seed, hipStream_t s)
          n->alloc(&n->gD2in, v1 * 96) && n->alloc(&n->gE3b, v2 * 64) && n->alloc(&n->gE3a, v2 * 64) &&
          n->alloc(&n->gP2, v2 * 32) && n->alloc(&n->gE2a, v1 * 32) && n->alloc(&n->gP1, v1 * 16) &&
          n->alloc(&n->gE1a, v0 * 16);
-    ok = ok && n->alloc(&n->part, maxparts * n->nsplit) && n->alloc(&n->lpart, nblk(v0) + 1) &&
+    ok = ok && n->alloc(&n->part, std::max(maxparts * n->nsplit, wgparts)) && n->alloc(&n->lpart, nblk(v0) + 1) &&
+         n->alloc(&n->wxs, wgx) && n->alloc(&n->wgs, wgg) && n->alloc(&n->part2, (long)WG_G * maxparts) &&
          n->alloc(&n->feat, (long)(max_samples + 1) * v0 * 4) && n->alloc(&n->tgt_raw, (long)(max_samples + 1) * v0 * 4) &&
          n->alloc(&n->tgt, v0 * 4) && n->alloc(&n->scale, NET_NOUT);
     if (!ok) {
@@ -913,6 +1090,13 @@ int net_conv_test(int R, int ntap, int cin, int cout, int relu, int mode, const 
         if (ntap == 27) hipLaunchKernelGGL((k_conv<27, 1, false>), grid, dim3(256), 0, s, R, bgo, cout, 0, cout / 16, Wq, (const float *)nullptr, (void *)dout, cin, 0, cin / 16);
         else hipLaunchKernelGGL((k_conv<1, 1, false>), grid, dim3(256), 0, s, R, bgo, cout, 0, cout / 16, Wq, (const float *)nullptr, (void *)dout, cin, 0, cin / 16);
         hipMemcpy(out, dout, nv * cin * 4, hipMemcpyDeviceToHost);
+    } else if (wg_staged_ok(R)) {
+        bf16 *xs = (bf16 *)A(wg_xs_elems(R, cin) * 2), *gs = (bf16 *)A(nv * cout * 2);
+        float *wpart = (float *)A(wg_plan(R, cin, cout).nsplit * nw * 4);
+        float *wpart2 = (float *)A((long)WG_G * (nw > cout ? nw : cout) * 4);
+        wgrad_staged(R, ntap, cin, cout, bin, cin, 0, bgo, xs, gs, wpart, wpart2, dgw, dgb, s);
+        hipMemcpy(gw, dgw, nw * 4, hipMemcpyDeviceToHost);
+        hipMemcpy(gb, dgb, cout * 4, hipMemcpyDeviceToHost);
     } else {
         long chunk = (nv + nsplit - 1) / nsplit;
         chunk = (chunk + 15) / 16 * 16;
