@@ -1117,15 +1117,15 @@ __global__ void __launch_bounds__(256) k_absdiff_max(long n, const double *a, co
 // ----------------------------------------------------------------- BLAS-1
 
 // sum of partials in fixed order (one block)
-__global__ void __launch_bounds__(1024) k_sum(int np, int nvec, const double *part, double *out)
+// out[v] = sum of the np partials of vector v; one workgroup per vector (grid nvec)
+__global__ void __launch_bounds__(1024) k_sum(int np, const double *part, double *out)
 {
     __shared__ double sh[16];
-    for (int v = 0; v < nvec; v++) {
-        double a = 0.0;
-        for (int q = threadIdx.x; q < np; q += blockDim.x) a += part[(long)v * np + q];
-        double s = block_sum(a, sh);
-        if (threadIdx.x == 0) out[v] = s;
-    }
+    const int v = blockIdx.x;
+    double a = 0.0;
+    for (int q = threadIdx.x; q < np; q += blockDim.x) a += part[(long)v * np + q];
+    const double s = block_sum(a, sh);
+    if (threadIdx.x == 0) out[v] = s;
 }
 
 template <typename TA, typename TB>
@@ -1476,7 +1476,8 @@ void launch_absdiff_max(long n, const double *a, const double *b, double *part, 
 }
 void launch_sum(int np, int nvec, const double *part, double *out, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_sum, dim3(1), dim3(1024), 0, s, np, nvec, part, out);
+    if (nvec < 1) return;
+    hipLaunchKernelGGL(k_sum, dim3((unsigned)nvec), dim3(1024), 0, s, np, part, out);
 }
 unsigned blas_blocks(long n) { return nb_grid(n); }
 template <typename TA, typename TB>
