@@ -149,17 +149,21 @@ __global__ void __launch_bounds__(64) k_conv_wgrad(int R, const bf16 *__restrict
     for (int q = 0; q < 4; q++) o[(long)q * Cout] = acc[q];
 }
 
-// Weight gradient from channel-first staged operands (R % 4 == 0).  The MFMA wants
-// four consecutive voxels of one channel per lane (K = voxels); the activations and
-// gradients are channel-last, so both are staged once per layer:
-//   xs[cb][s][ci][(i+1) P + (j+1)][k] = x(i, j, k + s - 1) of channel cb*16 + ci, zero
-//     outside the grid (P = R + 2: a zero row and plane all round; s: the three k shifts)
-//   gs[co][v] = gz(v, co)
-// so every operand quad is one aligned 8-byte load.  One wave accumulates all taps of
-// one (ci block, co block) over its voxel groups and writes its partial.
+// Weight gradient from staged operands (R % 4 == 0).  The MFMA wants four consecutive
+// voxels of one channel per lane (K = voxels, M = 16 channels); the activations and
+// gradients are channel-last, so both are staged once per layer with the 16 channels
+// of a block interleaved per quad of voxels:
+//   xs[cb][s][(i+1) P + (j+1)][k/4][ci][k%4] = x(i, j, k + s - 1) of channel cb*16 + ci,
+//     zero outside the grid (P = R + 2: a zero row and plane all round; s: the three
+//     k shifts)
+//   gs[ob][v/4][co%16][v%4] = gz(v, ob*16 + co%16)
+// so the operand of lane l = 16 qd + m for the 16 voxels from v0 (one k row) is the
+// 8 bytes at 64 (v0/4 + qd) + 4 m: a wave's load is 512 contiguous bytes.  One wave
+// accumulates all taps of one (ci block, co block) over its voxel groups and writes
+// its partial.
 // one (i+1, j+1) row of xs for one ci block per workgroup (grid: P*P rows x ncib): the
 // row's R voxels x 16 channels are read channel-contiguous into LDS with a zero at
-// each end, then written as the three k-shifted rows of every channel; pad rows are zero
+// each end, then written as the three k-shifted interleaved rows; pad rows are zero
 __global__ void __launch_bounds__(256) k_wg_stage_x(int R, const bf16 *__restrict__ x, int csx, int cix, int ncib,
                                                     bf16 *__restrict__ xs)
 {
@@ -179,55 +183,71 @@ __global__ void __launch_bounds__(256) k_wg_stage_x(int R, const bf16 *__restric
         }
     }
     __syncthreads();
-    const long plane = (long)P * P * R;
-    bf16 *o = xs + (long)cb * 3 * 16 * plane + (long)row * R;
+    const long rowl = 16L * R, plane = (long)P * P * rowl;
+    bf16 *o = xs + (long)cb * 3 * plane + (long)row * rowl;
     for (int t = threadIdx.x; t < 3 * 16 * R; t += blockDim.x) {
-        const int k = t % R, ci = (t / R) % 16, sh = t / (16 * R);
+        const int u = t % (16 * R), sh = t / (16 * R);
+        const int k = ((u >> 6) << 2) | (u & 3), ci = (u >> 2) & 15;
         bf16 v;
         v.v = pad ? (uint16_t)0 : sm[ci][k + sh];
-        o[((long)sh * 16 + ci) * plane + k] = v;
+        o[(long)sh * plane + u] = v;
     }
+}
+
+// position of gz(v, co) in the staged gradient (C % 16 == 0, nv % 4 == 0)
+__device__ __forceinline__ long gs_at(long nv, long v, int co)
+{
+    return (((long)(co >> 4) * (nv >> 2) + (v >> 2)) * 16 + (co & 15)) * 4 + (v & 3);
 }
 
 __global__ void __launch_bounds__(256) k_wg_stage_g(long nv, int C, const bf16 *__restrict__ gz, bf16 *__restrict__ gs)
 {
     const long tot = nv * C;
     for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < tot; q += (long)gridDim.x * blockDim.x) {
-        const int co = (int)(q / nv);
-        const long v = q % nv;
+        // q = ((ob (nv/4) + v/4) 16 + co%16) 4 + v%4
+        const long v = ((q >> 6) % (nv >> 2)) * 4 + (q & 3);
+        const int co = (int)(q / (16 * nv)) * 16 + (int)((q >> 2) & 15);
         gs[q] = gz[v * C + co];
     }
 }
 
-// grid (chunks, ci blocks x co blocks), 4 waves per block, gpw voxel groups of 16 per
-// wave; part[split][t][ci][co], split = chunk * 4 + wave (every entry of every split
-// is written, zeros for a wave past the last group)
+// 1D grid of nch chunks x (ci blocks x co blocks) pairs, a multiple of 8 workgroups,
+// 4 waves per block, gpw voxel groups of 16 per wave; part[split][t][ci][co], split =
+// chunk * 4 + wave (every entry of every split is written, zeros for a wave past the
+// last group).  Workgroup L runs on XCD L % 8; it takes logical block
+// (L % 8) * (nblocks / 8) + L / 8, so each XCD sweeps a contiguous run of chunks and
+// the rows that neighbouring chunks share (taps one row or plane away) meet in its L2.
 template <int NTAP>
 __global__ void __launch_bounds__(256) k_wg_staged(int R, const bf16 *__restrict__ xs, const bf16 *__restrict__ gs,
-                                                   int ncib, int ncob, long gpw, float *__restrict__ part)
+                                                   int ncib, int ncob, int nch, long gpw, float *__restrict__ part)
 {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int cb = blockIdx.y / ncob, ob = blockIdx.y % ncob;
+    const int L = blockIdx.x, per = gridDim.x >> 3;
+    const int lb = (L & 7) * per + (L >> 3);
+    const int chunk = lb % nch, pair = lb / nch;
+    const int cb = pair / ncob, ob = pair % ncob;
     const long nv = (long)R * R * R, ngroups = nv / 16;
     const int P = R + 2;
     const int m = lane & 15, qd = lane >> 4;
-    const bf16 *xb = xs + (long)cb * 3 * 16 * P * P * R;
-    const bf16 *gb = gs + (long)(ob * 16 + m) * nv;
+    const long rowl = 16L * R, plane = (long)P * P * rowl;
+    const bf16 *xb = xs + (long)cb * 3 * plane + 4 * m;
+    const bf16 *gb = gs + (long)ob * 16 * nv + 4 * m;
     f4 acc[NTAP];
 #pragma unroll
     for (int t = 0; t < NTAP; t++) acc[t] = f4{0.f, 0.f, 0.f, 0.f};
-    const long split = (long)blockIdx.x * 4 + w;
+    const long split = (long)chunk * 4 + w;
     const long g0 = split * gpw;
     const long g1 = (g0 + gpw < ngroups) ? g0 + gpw : ngroups;
     for (long g = g0; g < g1; g++) {
         const long v = g * 16 + 4 * qd;  // this lane's quad of voxels (one k row: R % 4 == 0)
         const int k = (int)(v % R), j = (int)((v / R) % R), i = (int)(v / ((long)R * R));
-        const bf4 bq = *(const bf4 *)(gb + v);  // B[k = voxel][n = co]
+        const bf4 bq = *(const bf4 *)(gb + 16 * v);  // B[k = voxel][n = co]
 #pragma unroll
         for (int t = 0; t < NTAP; t++) {
             int a, b, c;
             tap_off(t, NTAP, a, b, c);
-            const bf4 aq = *(const bf4 *)(xb + ((((long)(c + 1) * 16 + m) * P + (i + a + 1)) * P + (j + b + 1)) * R + k);
+            const long row = (long)(i + a + 1) * P + (j + b + 1);
+            const bf4 aq = *(const bf4 *)(xb + (c + 1) * plane + row * rowl + 16 * k);
             acc[t] = mfma16(as_s4(aq), as_s4(bq), acc[t]);  // A[m = ci][k = voxel]
         }
     }
@@ -251,15 +271,15 @@ __global__ void __launch_bounds__(256) k_reduce_parts_grouped(long n, int nsplit
     part2[(long)g * n + q] = s;
 }
 
-// bias gradient partials from the staged gradient gs[co][v]: block (g, co) sums its
-// segment of the row; part[g][co]
+// bias gradient partials from the staged gradient: block (g, co) sums its segment of
+// the voxels of channel co; part[g][co]
 __global__ void __launch_bounds__(256) k_rowsum_parts(long nv, const bf16 *__restrict__ gs, float *part)
 {
     __shared__ float sh[256];
     const int G = gridDim.x, g = blockIdx.x, co = blockIdx.y, C = gridDim.y;
     const long v0 = nv * g / G, v1 = nv * (g + 1) / G;
     float s = 0.f;
-    for (long v = v0 + threadIdx.x; v < v1; v += blockDim.x) s += to_f(gs[(long)co * nv + v]);
+    for (long v = v0 + threadIdx.x; v < v1; v += blockDim.x) s += to_f(gs[gs_at(nv, v, co)]);
     sh[threadIdx.x] = s;
     __syncthreads();
     for (int o = blockDim.x >> 1; o > 0; o >>= 1) {
@@ -590,6 +610,7 @@ WgPlan wg_plan(int R, int cin, int cout)
     long gpw = (ngroups + 4 * nch - 1) / (4 * nch);
     if (gpw < 1) gpw = 1;
     nch = (ngroups + 4 * gpw - 1) / (4 * gpw);
+    while ((nch * npairs) % 8) nch++;  // whole XCD runs (chunks past the last group write zeros)
     return {nch, gpw, nch * 4};
 }
 long wg_xs_elems(int R, int cin)
@@ -610,11 +631,12 @@ void wgrad_staged(int R, int ntap, int cin, int cout, const bf16 *x, int csx, in
     hipLaunchKernelGGL(k_wg_stage_x, dim3((unsigned)((R + 2) * (R + 2)), (unsigned)(cin / 16)), dim3(256), 0, s, R, x, csx,
                        cix, cin / 16, xs);
     hipLaunchKernelGGL(k_wg_stage_g, dim3(nblk_cap(nv * cout)), dim3(256), 0, s, nv, cout, gz, gs);
-    const dim3 grid((unsigned)pl.nchunks, (unsigned)((cin / 16) * (cout / 16)));
+    const dim3 grid((unsigned)(pl.nchunks * (cin / 16) * (cout / 16)));
+    const int nch = (int)pl.nchunks;
     if (ntap == 27)
-        hipLaunchKernelGGL((k_wg_staged<27>), grid, dim3(256), 0, s, R, xs, gs, cin / 16, cout / 16, pl.gpw, part);
+        hipLaunchKernelGGL((k_wg_staged<27>), grid, dim3(256), 0, s, R, xs, gs, cin / 16, cout / 16, nch, pl.gpw, part);
     else
-        hipLaunchKernelGGL((k_wg_staged<1>), grid, dim3(256), 0, s, R, xs, gs, cin / 16, cout / 16, pl.gpw, part);
+        hipLaunchKernelGGL((k_wg_staged<1>), grid, dim3(256), 0, s, R, xs, gs, cin / 16, cout / 16, nch, pl.gpw, part);
     hipLaunchKernelGGL(k_reduce_parts_grouped, dim3(nblk(nw), WG_G), dim3(256), 0, s, nw, (int)pl.nsplit, part, part2);
     hipLaunchKernelGGL(k_reduce_parts, dim3(nblk(nw)), dim3(256), 0, s, nw, WG_G, part2, gW);
     hipLaunchKernelGGL(k_rowsum_parts, dim3(WG_G, cout), dim3(256), 0, s, nv, gs, part2);
