@@ -65,9 +65,9 @@ struct Sten {
         if (DIM == 2 && d == 2) return C(0);
         const long q = idc(i, j, k);
         const C v = ldx((long)(1 + d) * g.S + q);
-        if (!in(i, j, k)) return C(0);
-        if (MODE != 2 && !((fl[q] >> (1 + d)) & 1)) return C(0);
-        return v;
+        const int f = fl[q];  // loaded whatever the tests give: no branch around the loads
+        const bool on = in(i, j, k) && (MODE == 2 || ((f >> (1 + d)) & 1));
+        return on ? v : C(0);
     }
     __device__ __forceinline__ C dv(int i, int j, int k) const
     {
@@ -75,7 +75,11 @@ struct Sten {
         if (DIM == 3) s += W(2, i, j, k + 1) - W(2, i, j, k);
         return s;
     }
-    __device__ __forceinline__ C gdiv(int i, int j, int k) const { return (F(i, j, k) & 1) ? dv(i, j, k) : C(0); }
+    __device__ __forceinline__ C gdiv(int i, int j, int k) const
+    {
+        const C d = dv(i, j, k);
+        return (F(i, j, k) & 1) ? d : C(0);
+    }
     __device__ __forceinline__ C ex(int i, int j, int k) const
     { return W(1, i, j, k) - W(1, i, j, k - 1) - W(2, i, j, k) + W(2, i, j - 1, k); }
     __device__ __forceinline__ C ey(int i, int j, int k) const
@@ -99,23 +103,26 @@ struct Sten {
         }
         return r;
     }
+    // Aface and Acell evaluate both rows (free and fixed) and select by the flag, so
+    // that no load waits for the flag
     template <int d>
     __device__ __forceinline__ C Aface(int i, int j, int k) const
     {
         long q = id(i, j, k);
-        C u = ldx((long)(1 + d) * g.S + q);
-        if (!((fl[q] >> (1 + d)) & 1)) return u;  // definite fix / padding
+        const C u = ldx((long)(1 + d) * g.S + q);
+        const int f = fl[q];
         const int di = (d == 0), dj = (d == 1), dk = (d == 2);
         C r = visc<d>(i, j, k);
-        r += X(0, i - di, j - dj, k - dk) - ldx(q);
+        r += Xb(0, i - di, j - dj, k - dk) - ldx(q);
         r += Cvt<TS>::ld(al[(long)d * g.S + q]) * u;
-        return r;
+        return ((f >> (1 + d)) & 1) ? r : u;  // definite fix / padding: u
     }
     __device__ __forceinline__ C Acell(int i, int j, int k) const
     {
         long q = id(i, j, k);
-        if (fl[q] & 1) return dv(i, j, k);
-        return ldx(q);
+        const C dvc = dv(i, j, k);
+        const C p = ldx(q);
+        return (fl[q] & 1) ? dvc : p;
     }
     // objective operator G of FT/FluidEnergyGrad3D.cu:27-134 (no pressure terms, no fix)
     template <int d>
