@@ -33,6 +33,7 @@ struct Lvl {
     int Px, Py, Pz;        // padded node box
     long S;                // nodes per field
     int nfield;            // 1 + dim
+    int tiled;             // may run the 2.5D tiled kernels (when its shape allows them)
 };
 
 static inline Lvl make_lvl(int dim, int nx, int ny, int nz)
@@ -43,6 +44,7 @@ static inline Lvl make_lvl(int dim, int nx, int ny, int nz)
     if (dim == 3 && nz >= 64) g.Pz = (g.Pz + ST_ROW_ALIGN - 1) / ST_ROW_ALIGN * ST_ROW_ALIGN;
     g.S = (long)g.Px * g.Py * g.Pz;
     g.nfield = 1 + dim;
+    g.tiled = 1;
     return g;
 }
 

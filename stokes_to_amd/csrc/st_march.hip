@@ -450,7 +450,10 @@ __global__ void __launch_bounds__(BJ * 64) k_march(Lvl g, const uint8_t *__restr
     }
 }
 
-bool march_ok(const Lvl &g) { return g.dim == 3 && g.nz >= 64 && g.nz % BK == 0 && g.ny % 4 == 0 && g.nx >= 8; }
+bool march_ok(const Lvl &g)
+{
+    return g.tiled && g.dim == 3 && g.nz >= 64 && g.nz % BK == 0 && g.ny % 4 == 0 && g.nx >= 8;
+}
 
 // grid of BJ-row tiles x i-chunks: at least `target` workgroups, at least 8 planes
 // per chunk

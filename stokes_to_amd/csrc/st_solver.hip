@@ -696,6 +696,10 @@ int st_create(st_ctx **out, const st_params *p)
         if (m % 2 != 0 || m / 2 < 2) break;  // coarsest level 2^dim cells
         m /= 2;
     }
+    // a coarse level below 128 cells a side runs the per-cell gather kernels: with
+    // few planes per tile the marching kernels are latency-bound and slower there
+    for (size_t l = 1; l < c->lv.size(); l++)
+        if (c->lv[l].nz < 128) c->lv[l].tiled = 0;
     c->L = (int)c->lv.size();
     if (c->L > ST_MAX_LEVELS) {
         st_destroy(c);
