@@ -72,8 +72,9 @@ void launch_lift(const Lvl &, const uint8_t *fl, const double *xd, const double 
 void launch_energy(const Lvl &, const uint8_t *fl, const double *al, const double *x, double *y, double *part,
                    hipStream_t);
 unsigned energy_blocks(const Lvl &);
-void launch_filter(const Lvl &, double R, const double *W, const double *in, double *out, int transpose, hipStream_t);
-void launch_filter_w(const Lvl &, double R, double *W, hipStream_t);
+// Wi: reciprocal weight sums of the cells, written by launch_filter_w
+void launch_filter(const Lvl &, double R, const double *Wi, const double *in, double *out, int transpose, hipStream_t);
+void launch_filter_w(const Lvl &, double R, double *Wi, hipStream_t);
 constexpr int OC_K = 16;  // multipliers per pass of the OC search
 struct OcFactors {
     double f[OC_K];

@@ -1007,7 +1007,11 @@ __global__ void __launch_bounds__(256) k_energy(Lvl g, const uint8_t *fl, const 
 // hat filter weights max(0, R - |o|), neighbours inside the domain only:
 // W_i = sum_j w_ij (row weight), forward out_i = sum_j w_ij in_j / W_i,
 // transpose out_j = sum_i w_ij in_i / W_i (H symmetric)
-__global__ void __launch_bounds__(256) k_filter_w(int dim, int nx, int ny, int nz, int r, double R, double *W)
+// Density filter (the reference's cone filter: weight R - |o| for the neighbours o
+// with |o| < R, normalised by the weight sum W of the cell).  Wi = 1 / W per cell is
+// computed once per context; the forward filter is (sum h in) Wi, its transpose
+// sum h in Wi over the neighbours.
+__global__ void __launch_bounds__(256) k_filter_w(int dim, int nx, int ny, int nz, int r, double R, double *Wi)
 {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     long nc = (long)nx * ny * nz;
@@ -1026,32 +1030,95 @@ __global__ void __launch_bounds__(256) k_filter_w(int dim, int nx, int ny, int n
                 double h = R - sqrt((double)(a * a + b * b + c * c));
                 if (h > 0) w += h;
             }
-    W[t] = w;
+    Wi[t] = 1.0 / w;
 }
 
-__global__ void __launch_bounds__(256) k_filter(int dim, int nx, int ny, int nz, int r, double R, const double *W,
-                                                const double *in, double *out, int transpose)
+// neighbour weights by squared offset length, up to radius FILT_RT.  RR > 0: the
+// filter's half-width r = RR and the dimension D are compile-time (the taps unroll and
+// their loads are issued together; cell index in 32 bits, nc < 2^32); RR = 0: runtime
+// r and dimension.
+constexpr int FILT_RT = 6;
+template <int RR, int D, bool TR>
+__global__ void __launch_bounds__(256) k_filter(int dim, int nx, int ny, int nz, int r, double R,
+                                                const double *__restrict__ Wi, const double *__restrict__ in,
+                                                double *__restrict__ out)
 {
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    long nc = (long)nx * ny * nz;
+    __shared__ double hw[3 * FILT_RT * FILT_RT + 1];
+    if (RR > 0) {
+        r = RR;
+        dim = D;
+    }
+    const int d2max = 3 * r * r;
+    if (r <= FILT_RT)
+        for (int q = threadIdx.x; q <= d2max; q += blockDim.x) {
+            const double h = R - sqrt((double)q);
+            hw[q] = (h > 0) ? h : 0.0;
+        }
+    __syncthreads();
+    // the grid is a multiple of 8 workgroups; workgroup L runs on XCD L % 8 and takes
+    // block (L % 8) (grid / 8) + L / 8, so each XCD sweeps a contiguous range of cells
+    // and the neighbour planes it re-reads stay in its L2
+    const long per = gridDim.x >> 3;
+    const long blk = (long)(blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    const long t = blk * blockDim.x + threadIdx.x;
+    const long nc = (long)nx * ny * nz;
     if (t >= nc) return;
-    int k = (int)(t % nz);
-    long tt = t / nz;
-    int j = (int)(tt % ny);
-    int i = (int)(tt / ny);
+    int i, j, k;
+    if (RR > 0) {
+        const unsigned tu = (unsigned)t, tt = tu / (unsigned)nz;
+        k = (int)(tu - tt * (unsigned)nz);
+        i = (int)(tt / (unsigned)ny);
+        j = (int)(tt - (unsigned)i * (unsigned)ny);
+    } else {
+        k = (int)(t % nz);
+        const long tt = t / nz;
+        j = (int)(tt % ny);
+        i = (int)(tt / ny);
+    }
     const int rz = (dim == 3) ? r : 0;
     double s = 0.0;
-    for (int a = -r; a <= r; a++)
-        for (int b = -r; b <= r; b++)
-            for (int c = -rz; c <= rz; c++) {
-                int ii = i + a, jj = j + b, kk = k + c;
-                if (ii < 0 || jj < 0 || kk < 0 || ii >= nx || jj >= ny || kk >= nz) continue;
-                double h = R - sqrt((double)(a * a + b * b + c * c));
-                if (h <= 0) continue;
-                long q = ((long)ii * ny + jj) * nz + kk;
-                s += transpose ? h * in[q] / W[q] : h * in[q];
-            }
-    out[t] = transpose ? s : s / W[t];
+    if (RR == 1) {
+        // per-axis offsets and validity, combined per tap: a neighbour outside the grid
+        // reads the cell itself with weight 0 (branch-free, 32-bit cell offsets)
+        const int A = ny * nz, B = nz;
+        const int oi[3] = {i > 0 ? -A : 0, 0, i < nx - 1 ? A : 0};
+        const int oj[3] = {j > 0 ? -B : 0, 0, j < ny - 1 ? B : 0};
+        const int ok[3] = {k > 0 ? -1 : 0, 0, k < nz - 1 ? 1 : 0};
+        const bool vi[3] = {i > 0, true, i < nx - 1}, vj[3] = {j > 0, true, j < ny - 1},
+                   vk[3] = {k > 0, true, k < nz - 1};
+        const unsigned tu = (unsigned)t;
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+#pragma unroll
+            for (int b = 0; b < 3; b++)
+#pragma unroll
+                for (int c = (D == 3 ? 0 : 1); c < (D == 3 ? 3 : 2); c++) {
+                    const int d2 = (a - 1) * (a - 1) + (b - 1) * (b - 1) + (c - 1) * (c - 1);
+                    const unsigned q = tu + (unsigned)(oi[a] + oj[b] + ok[c]);
+                    const double hv = (vi[a] && vj[b] && vk[c]) ? hw[d2] : 0.0;
+                    s += TR ? hv * in[q] * Wi[q] : hv * in[q];
+                }
+    } else {
+#pragma unroll
+        for (int a = -r; a <= r; a++)
+#pragma unroll
+            for (int b = -r; b <= r; b++)
+#pragma unroll
+                for (int c = -rz; c <= rz; c++) {
+                    const int d2 = a * a + b * b + c * c;
+                    double h;
+                    if (r <= FILT_RT) h = hw[d2];
+                    else {
+                        h = R - sqrt((double)d2);
+                        if (h < 0) h = 0.0;
+                    }
+                    const int ii = i + a, jj = j + b, kk = k + c;
+                    if (h == 0.0 || ii < 0 || jj < 0 || kk < 0 || ii >= nx || jj >= ny || kk >= nz) continue;
+                    const long q = ((long)ii * ny + jj) * nz + kk;
+                    s += TR ? h * in[q] * Wi[q] : h * in[q];
+                }
+    }
+    out[t] = TR ? s : s * Wi[t];
 }
 
 // OC in factored form: at multiplier l the update is rho * base * l^-eta clamped to the
@@ -1446,20 +1513,30 @@ void launch_energy(const Lvl &g, const uint8_t *fl, const double *al, const doub
     else hipLaunchKernelGGL((k_energy<2>), dim3(nb(g.S)), dim3(256), 0, s, g, fl, al, x, y, part);
 }
 unsigned energy_blocks(const Lvl &g) { return nb(g.S); }
-void launch_filter(const Lvl &g, double R, const double *W, const double *in, double *out, int transpose,
+void launch_filter(const Lvl &g, double R, const double *Wi, const double *in, double *out, int transpose,
                    hipStream_t s)
 {
     long nc = (long)g.nx * g.ny * g.nz;
     int r = (int)ceil(R) - 1;
     if (r < 0) r = 0;
-    hipLaunchKernelGGL(k_filter, dim3(nb(nc)), dim3(256), 0, s, g.dim, g.nx, g.ny, g.nz, r, R, W, in, out, transpose);
+    const dim3 gr((nb(nc) + 7) / 8 * 8), bl(256);
+#define ST_FILT(RR, D)                                                                                        \
+    do {                                                                                                      \
+        if (transpose) hipLaunchKernelGGL((k_filter<RR, D, true>), gr, bl, 0, s, g.dim, g.nx, g.ny, g.nz, r, R, Wi, in, out); \
+        else hipLaunchKernelGGL((k_filter<RR, D, false>), gr, bl, 0, s, g.dim, g.nx, g.ny, g.nz, r, R, Wi, in, out);        \
+    } while (0)
+    if (r == 1 && nc < (1L << 32)) {
+        if (g.dim == 3) ST_FILT(1, 3);
+        else ST_FILT(1, 2);
+    } else ST_FILT(0, 0);
+#undef ST_FILT
 }
-void launch_filter_w(const Lvl &g, double R, double *W, hipStream_t s)
+void launch_filter_w(const Lvl &g, double R, double *Wi, hipStream_t s)
 {
     long nc = (long)g.nx * g.ny * g.nz;
     int r = (int)ceil(R) - 1;
     if (r < 0) r = 0;
-    hipLaunchKernelGGL(k_filter_w, dim3(nb(nc)), dim3(256), 0, s, g.dim, g.nx, g.ny, g.nz, r, R, W);
+    hipLaunchKernelGGL(k_filter_w, dim3(nb(nc)), dim3(256), 0, s, g.dim, g.nx, g.ny, g.nz, r, R, Wi);
 }
 unsigned oc_grid(long n) { return nb_grid(n); }
 void launch_oc_base(long n, const double *dc, const double *dv, double eta, double *base, hipStream_t s)

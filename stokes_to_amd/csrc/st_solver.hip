@@ -90,6 +90,10 @@ struct st_ctx {
     // extrapolated start when it computed it; consumed or dropped by the next solve)
     bool r0_ready = false;
     double r0_beta = 0.0;
+    // multiplier of the last OC update (0: none), where the next search starts
+    double oc_lambda = 0.0;
+    // dv = dV/drho = H^T(1/(n W)) depends on the grid and the filter only: formed once
+    bool dv_ready = false;
     // kernel timing: event pairs around finest-level launches
     bool prof = false;
     std::vector<hipEvent_t> evpool[ST_PROF_NKIND];
@@ -1354,15 +1358,26 @@ int st_filter(st_ctx *c, const double *in, double *out, int transpose)
 }
 
 // OC with bisection on the device: rho (dev), dc (dev) -> rn (dev)
+// c->dv = dV/drho = H^T(1/(n W)), formed on first use
+static void volume_gradient(st_ctx *c)
+{
+    if (c->dv_ready) return;
+    launch_fill<double>(c->ncell, c->tmpc, 1.0 / (double)c->ncell, c->s);
+    filter_dev(c, c->tmpc, c->dv, 1);
+    c->dv_ready = true;
+}
+
 static int oc_dev(st_ctx *c, const double *rho, const double *dc, double volfrac, double move, double eta,
                   double rho_min, double *rn, double *change, double *vol_out)
 {
     const long n = c->ncell;
-    launch_fill<double>(n, c->tmpc, 1.0 / (double)n, c->s);
-    filter_dev(c, c->tmpc, c->dv, 1);  // dV/drho = H^T(1/(n W))
+    volume_gradient(c);
     // search for the multiplier: the volume falls with it; each pass evaluates OC_K
-    // multipliers spread over the bracket and keeps the sub-bracket where the volume
-    // crosses volfrac (17-fold narrowing per pass, the bisection's tolerance)
+    // multipliers and keeps the sub-bracket where the volume crosses volfrac, down to
+    // the bisection's tolerance.  The first pass spreads them geometrically over a
+    // factor 2^(+-3.75) around the previous update's multiplier (the multiplier moves
+    // little between design steps; if it moved further, the bracket becomes the side
+    // it lies on); the others evenly over the bracket (17-fold narrowing per pass).
     double *base = c->tmpc;  // free once dv is formed
     launch_oc_base(n, dc, c->dv, eta, base, c->s);
     const int gb = (int)oc_grid(n);
@@ -1370,8 +1385,10 @@ static int oc_dev(st_ctx *c, const double *rho, const double *dc, double volfrac
     for (int pass = 0; (l2 - l1) > 1e-13 * (l1 + l2) && pass < 60; pass++) {
         OcFactors f;
         double lk[OC_K], vk[OC_K];
+        const bool around = pass == 0 && c->oc_lambda > 0.0 && c->oc_lambda < 1e8;
         for (int k = 0; k < OC_K; k++) {
-            lk[k] = l1 + (l2 - l1) * (double)(k + 1) / (double)(OC_K + 1);
+            lk[k] = around ? c->oc_lambda * std::exp2(0.5 * ((double)k - 0.5 * (OC_K - 1)))
+                           : l1 + (l2 - l1) * (double)(k + 1) / (double)(OC_K + 1);
             f.f[k] = std::pow(lk[k], -eta);
         }
         launch_oc_vols(n, rho, base, c->dv, f, move, rho_min, c->part, c->s);
@@ -1384,6 +1401,7 @@ static int oc_dev(st_ctx *c, const double *rho, const double *dc, double volfrac
         l1 = n1;
         l2 = n2;
     }
+    c->oc_lambda = 0.5 * (l1 + l2);
     launch_oc_apply(n, rho, base, c->dv, std::pow(0.5 * (l1 + l2), -eta), move, rho_min, rn, c->part, c->s);
     launch_sum(gb, 1, c->part, c->red, c->s);
     CK(hipMemcpyAsync(&vol, c->red, sizeof(double), hipMemcpyDeviceToHost, c->s));
@@ -1462,8 +1480,7 @@ static int mma_dev(st_ctx *c, const double *rho, const double *dc, double volfra
 {
     const long n = c->ncell;
     CKR(mma_ready(c));
-    launch_fill<double>(n, c->tmpc, 1.0 / (double)n, c->s);
-    filter_dev(c, c->tmpc, c->dv, 1);  // dV/drho
+    volume_gradient(c);
     double vol = 0.0;
     launch_dot<double, double>(n, rho, c->dv, c->part, c->s);
     launch_sum((int)blas_blocks(n), 1, c->part, c->red, c->s);
