@@ -66,6 +66,8 @@ def parse():
     ap.add_argument("--mp", default="fp32", choices=["fp32", "bf16"])
     ap.add_argument("--nu", type=int, nargs=2, default=(2, 2), metavar=("PRE", "POST"),
                     help="smoothing sweeps of the V-cycle (all contexts)")
+    ap.add_argument("--warm-order", type=int, default=None, choices=[0, 1, 2, 3],
+                    help="start of a sequence of solves (st_params.warm_order; default: the library's)")
     ap.add_argument("--epochs", type=int, default=3)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -83,7 +85,8 @@ def log(args, *a):
 
 def new_ctx(args, prob, precision):
     from stokes_to_amd.steady import FluidSteadyState
-    s = FluidSteadyState(3, args.n, precision=precision, rtol=args.rtol, nu_pre=args.nu[0], nu_post=args.nu[1])
+    kw = {} if args.warm_order is None else {"warm_order": args.warm_order}
+    s = FluidSteadyState(3, args.n, precision=precision, rtol=args.rtol, nu_pre=args.nu[0], nu_post=args.nu[1], **kw)
     s.init_boundary(prob.fixed)
     s.update_b(dirichlet=prob.dirichlet)
     return s

@@ -65,8 +65,11 @@ typedef struct {
                             faster on MI355X (the one-pass kernel holds 126 KB of LDS) */
     int warm_order;      /* ST_INIT_KEEP / ST_INIT_LEARNED starts of a sequence of solves
                             (st_to_step, repeated st_solve): 0 the held solution; 1 its linear,
-                            2 (default) its quadratic extrapolation from the previous solutions of
-                            the sequence, taken when its true FP64 residual is lower */
+                            2 its quadratic extrapolation from the previous solutions of the
+                            sequence, taken when its true FP64 residual is lower; 3 (default) the
+                            held solution plus the combination of the last three differences of
+                            solutions with the least residual (FP32 precision on a tiled finest
+                            level; elsewhere 2) */
 } st_params;
 
 typedef struct {

@@ -97,6 +97,18 @@ void launch_multiaxpy(long n, TW *w, const TV *V, long ld, int m, const double *
                       hipStream_t);
 template <typename TX, typename TY> void launch_axpby(long n, double a, const TX *x, double b, TY *y, hipStream_t);
 void launch_lincomb3(long n, double a, const double *x, double b, const double *w, double c, double *y, hipStream_t);
+// minimal-residual start: d = float(a - b); Gram of nd <= 3 FP32 vectors y_i (stride ld)
+// and their products with r (partials: nd(nd+1)/2 + nd per block, mr_blocks(n) blocks);
+// x0 = x1 + sum d_i D_i, r -= sum d_i y_i with the partials of the new ||r||^2
+struct MrCoef {
+    double d[3];
+};
+void launch_diff_f32(long n, const double *a, const double *b, float *d, hipStream_t);
+unsigned mr_blocks(long n);
+void launch_mr_gram(int nd, long n, const float *y, long ld, const float *r, double *part, hipStream_t);
+void launch_mr_update(int nd, long n, const double *x1, const float *const *D, const float *y, long ld,
+                      const MrCoef &cf, double *x0, float *r, double *part, hipStream_t);
+void march_apply_f32(const Lvl &, const uint8_t *, const double *al, const float *x, float *y, hipStream_t);
 template <typename TS> void launch_fill(long n, TS *y, double v, hipStream_t);
 
 // 2.5D marching kernels (st_march.hip) for 3D levels with ny % 4 == 0, nz % 64 == 0

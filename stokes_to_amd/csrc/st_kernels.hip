@@ -1379,6 +1379,68 @@ __global__ void __launch_bounds__(256) k_lincomb3(long n, double a, const double
     }
 }
 
+// Minimal-residual start (see st_solver.hip): d = float(a - b); the Gram matrix of the
+// images y_i = A D_i and their products with the residual r in one pass; the update
+// x0 = x1 + sum d_i D_i, r <- r - sum d_i y_i with the new ||r||^2.
+__global__ void __launch_bounds__(256) k_diff_f32(long n, const double *a, const double *b, float *d)
+{
+    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x)
+        d[q] = (float)(a[q] - b[q]);
+}
+
+// part[k][block], k over G_ij (i <= j < ND, row by row) then g_i = y_i . r
+template <int ND>
+__global__ void __launch_bounds__(256) k_mr_gram(long n, const float *y, long ld, const float *r, double *part)
+{
+    __shared__ double sh[16];
+    constexpr int NQ = ND * (ND + 1) / 2 + ND;
+    double acc[NQ];
+#pragma unroll
+    for (int k = 0; k < NQ; k++) acc[k] = 0.0;
+    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+        double v[ND];
+#pragma unroll
+        for (int i = 0; i < ND; i++) v[i] = (double)y[i * ld + q];
+        const double rq = (double)r[q];
+        int k = 0;
+#pragma unroll
+        for (int i = 0; i < ND; i++)
+#pragma unroll
+            for (int j = i; j < ND; j++) acc[k++] += v[i] * v[j];
+#pragma unroll
+        for (int i = 0; i < ND; i++) acc[k++] += v[i] * rq;
+    }
+#pragma unroll
+    for (int k = 0; k < NQ; k++) {
+        const double t = block_sum(acc[k], sh);
+        if (threadIdx.x == 0) part[(long)k * gridDim.x + blockIdx.x] = t;
+    }
+}
+
+template <int ND>
+__global__ void __launch_bounds__(256) k_mr_update(long n, const double *x1, const float *D0, const float *D1,
+                                                   const float *D2, const float *y, long ld, MrCoef cf, double *x0,
+                                                   float *r, double *part)
+{
+    __shared__ double sh[16];
+    const float *D[3] = {D0, D1, D2};
+    double acc = 0.0;
+    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+        double xv = x1[q], rv = (double)r[q];
+#pragma unroll
+        for (int i = 0; i < ND; i++) {
+            xv += cf.d[i] * (double)D[i][q];
+            rv -= cf.d[i] * (double)y[i * ld + q];
+        }
+        x0[q] = xv;
+        const float rf = (float)rv;
+        r[q] = rf;
+        acc += (double)rf * (double)rf;
+    }
+    const double t = block_sum(acc, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
 template <typename TS>
 __global__ void __launch_bounds__(256) k_fill(long n, TS *y, double v)
 {
@@ -1609,6 +1671,29 @@ void launch_multiaxpy(long n, TW *w, const TV *V, long ld, int m, const double *
 void launch_lincomb3(long n, double a, const double *x, double b, const double *w, double c, double *y, hipStream_t s)
 {
     hipLaunchKernelGGL(k_lincomb3, dim3(nb_grid(n)), dim3(256), 0, s, n, a, x, b, w, c, y);
+}
+void launch_diff_f32(long n, const double *a, const double *b, float *d, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_diff_f32, dim3(nb_grid(n)), dim3(256), 0, s, n, a, b, d);
+}
+unsigned mr_blocks(long n) { return nb_grid(n); }
+void launch_mr_gram(int nd, long n, const float *y, long ld, const float *r, double *part, hipStream_t s)
+{
+    const dim3 g(nb_grid(n)), b(256);
+    if (nd == 1) hipLaunchKernelGGL((k_mr_gram<1>), g, b, 0, s, n, y, ld, r, part);
+    else if (nd == 2) hipLaunchKernelGGL((k_mr_gram<2>), g, b, 0, s, n, y, ld, r, part);
+    else hipLaunchKernelGGL((k_mr_gram<3>), g, b, 0, s, n, y, ld, r, part);
+}
+void launch_mr_update(int nd, long n, const double *x1, const float *const *D, const float *y, long ld,
+                      const MrCoef &cf, double *x0, float *r, double *part, hipStream_t s)
+{
+    const dim3 g(nb_grid(n)), b(256);
+    const float *D0 = D[0], *D1 = nd > 1 ? D[1] : D[0], *D2 = nd > 2 ? D[2] : D[0];
+    if (nd == 1)
+        hipLaunchKernelGGL((k_mr_update<1>), g, b, 0, s, n, x1, D0, D1, D2, y, ld, cf, x0, r, part);
+    else if (nd == 2)
+        hipLaunchKernelGGL((k_mr_update<2>), g, b, 0, s, n, x1, D0, D1, D2, y, ld, cf, x0, r, part);
+    else hipLaunchKernelGGL((k_mr_update<3>), g, b, 0, s, n, x1, D0, D1, D2, y, ld, cf, x0, r, part);
 }
 template <typename TX, typename TY>
 void launch_axpby(long n, double a, const TX *x, double b, TY *y, hipStream_t s)

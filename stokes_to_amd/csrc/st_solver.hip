@@ -90,6 +90,9 @@ struct st_ctx {
     // extrapolated start when it computed it; consumed or dropped by the next solve)
     bool r0_ready = false;
     double r0_beta = 0.0;
+    // differences of consecutive solutions of the forward (DX) and adjoint (DL)
+    // sequences in FP32, newest first, for the minimal-residual start (warm_order 3)
+    float *DX[3] = {nullptr, nullptr, nullptr}, *DL[3] = {nullptr, nullptr, nullptr};
     // multiplier of the last OC update (0: none), where the next search starts
     double oc_lambda = 0.0;
     // dv = dV/drho = H^T(1/(n W)) depends on the grid and the filter only: formed once
@@ -663,7 +666,7 @@ void st_default_params(st_params *p, int dim, int n)
     p->filter_radius = 1.5;
     p->optimizer = ST_OPT_OC;
     p->fused_sweep = 0;
-    p->warm_order = 2;
+    p->warm_order = 3;
 }
 
 int st_create(st_ctx **out, const st_params *p)
@@ -680,7 +683,7 @@ int st_create(st_ctx **out, const st_params *p)
     }
     if (p->restart < 1 || p->restart > 32 || p->max_iter < 1 || !(p->rtol > 0)) return ST_EINVAL;
     if (p->precision < ST_FP64 || p->precision > ST_BF16) return ST_EINVAL;
-    if (p->warm_order < 0 || p->warm_order > 2) return ST_EINVAL;
+    if (p->warm_order < 0 || p->warm_order > 3) return ST_EINVAL;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return ST_EHIP;
     st_ctx *c = new st_ctx();
@@ -1002,11 +1005,133 @@ static int solve_core_(st_ctx *c, const double *b, double *x, const double *xd, 
 static int resid_norm(st_ctx *c, const double *b, const double *x, double *nrm);
 static int resid_norm_u0(st_ctx *c, const double *b, const double *x, double *nrm);
 
-static int warm_extrapolate(st_ctx *c, const double *b, double **xp, double **h, int &nh, float *ms)
+// Minimal-residual start (warm_order 3, FP32 path on a tiled finest level): with x1 the
+// held solution and D_i the differences of the last solutions of the sequence (up to
+// 3, newest first, D_1 = x1 - x_2 formed here), the start is
+//   x0 = x1 + sum d_i D_i,  d = argmin || r1 - sum d_i A D_i ||,  r1 = b - A x1,
+// taken when its residual is below ||r1||.  The span holds x1 itself and both
+// polynomial extrapolations (2 x1 - x2 = x1 + D_1, 3 x1 - 3 x2 + x3 = x1 + 2 D_1 - D_2),
+// so the start is never worse than any of them.  r1 is the FP64 residual written in FP32
+// into u_0; A D_i (FP64 operator, FP32 vectors) go into the next three basis slots
+// (free until the solve); d from the normal equations of their Gram matrix (one pass);
+// the update writes x0 and r0 = r1 - sum d_i A D_i into u_0, which the solve takes as its
+// first residual.  The corrections are small next to x1, so the FP32 differences and
+// products change x0 by ~1e-7 of the correction only.
+static int warm_mr(st_ctx *c, const double *b, double **xp, double **h, int &nh, float **dh, float *ms)
+{
+    const long nv = c->nvec;
+    const Lvl &g = c->lv[0];
+    if (!h[0]) {
+        CKR(dalloc(c, &h[0], nv));
+        CKR(dalloc(c, &h[1], nv));
+    }
+    if (!dh[0])
+        for (int i = 0; i < 3; i++) {
+            double *t = nullptr;
+            CKR(dalloc(c, &t, (nv + 1) / 2));
+            dh[i] = reinterpret_cast<float *>(t);
+        }
+    hipEvent_t a, e;
+    CK(hipEventCreate(&a));
+    CK(hipEventCreate(&e));
+    CK(hipEventRecord(a, c->s));
+    double *x1 = *xp, *f = h[1];
+    if (nh >= 1) {  // newest difference x1 - x_2 into the oldest slot, then rotate
+        float *t = dh[2];
+        dh[2] = dh[1];
+        dh[1] = dh[0];
+        dh[0] = t;
+        launch_diff_f32(nv, x1, h[0], dh[0], c->s);
+    }
+    const int nd = std::min(nh, 3);
+    float *u0 = reinterpret_cast<float *>(c->V);
+    double r1 = 0.0;
+    CKR(resid_norm_u0(c, b, x1, &r1));
+    double r0 = r1;
+    bool take = false;
+    if (nd >= 1 && std::isfinite(r1) && r1 > 0.0) {
+        float *y = u0 + nv;  // basis slots 1..nd
+        for (int i = 0; i < nd; i++) march_apply_f32(g, c->fl[0], c->h64.al[0], dh[i], y + i * nv, c->s);
+        const int nq = nd * (nd + 1) / 2 + nd;
+        const int nbk = (int)mr_blocks(nv);
+        launch_mr_gram(nd, nv, y, nv, u0, c->part, c->s);
+        launch_sum(nbk, nq, c->part, c->red, c->s);
+        double q[9];
+        CK(hipMemcpyAsync(q, c->red, sizeof(double) * nq, hipMemcpyDeviceToHost, c->s));
+        CK(hipStreamSynchronize(c->s));
+        // G d = g (G symmetric positive semi-definite) by elimination in order; a pivot
+        // below 1e-12 of the largest diagonal marks a (near-)dependent direction, which
+        // gets a zero coefficient
+        double G[3][3] = {}, rhs[3] = {};
+        int k = 0;
+        for (int i = 0; i < nd; i++)
+            for (int j = i; j < nd; j++) G[i][j] = G[j][i] = q[k++];
+        for (int i = 0; i < nd; i++) rhs[i] = q[k++];
+        double scale = 0.0;
+        for (int i = 0; i < nd; i++) scale = std::max(scale, G[i][i]);
+        bool ok = scale > 0.0 && std::isfinite(scale), drop[3] = {false, false, false};
+        for (int p = 0; ok && p < nd; p++) {
+            if (!(G[p][p] > 1e-12 * scale)) {
+                drop[p] = true;
+                continue;
+            }
+            for (int i = p + 1; i < nd; i++) {
+                const double m = G[i][p] / G[p][p];
+                for (int j = p; j < nd; j++) G[i][j] -= m * G[p][j];
+                rhs[i] -= m * rhs[p];
+            }
+        }
+        MrCoef cf{{0.0, 0.0, 0.0}};
+        for (int p = nd - 1; ok && p >= 0; p--) {
+            if (drop[p]) continue;
+            double t = rhs[p];
+            for (int j = p + 1; j < nd; j++) t -= G[p][j] * cf.d[j];
+            cf.d[p] = t / G[p][p];
+            if (!std::isfinite(cf.d[p])) ok = false;
+        }
+        double gd = 0.0;
+        for (int i = 0; ok && i < nd; i++) gd += cf.d[i] * q[nd * (nd + 1) / 2 + i];
+        // predicted ||r0||^2 = ||r1||^2 - g.d (d solves the normal equations)
+        if (ok && gd > 1e-6 * r1 * r1) {
+            launch_mr_update(nd, nv, x1, dh, y, nv, cf, f, u0, c->part, c->s);
+            launch_sum(nbk, 1, c->part, c->red, c->s);
+            double rr = 0.0;
+            CK(hipMemcpyAsync(&rr, c->red, sizeof(double), hipMemcpyDeviceToHost, c->s));
+            CK(hipStreamSynchronize(c->s));
+            r0 = std::sqrt(rr);
+            take = std::isfinite(r0) && r0 < r1;
+        }
+    }
+    if (take) {
+        *xp = f;
+        h[1] = h[0];
+        h[0] = x1;
+        c->r0_ready = true;
+        c->r0_beta = r0;
+    } else {
+        // x1 stays the start; u_0 holds r1 unless the update overwrote it
+        launch_axpby<double, double>(nv, 1.0, x1, 0.0, f, c->s);
+        h[1] = h[0];
+        h[0] = f;
+        c->r0_ready = (r0 == r1);
+        c->r0_beta = r1;
+    }
+    nh = std::min(nh + 1, 3);
+    CK(hipEventRecord(e, c->s));
+    CK(hipEventSynchronize(e));
+    hipEventElapsedTime(ms, a, e);
+    hipEventDestroy(a);
+    hipEventDestroy(e);
+    return ST_OK;
+}
+
+static int warm_extrapolate(st_ctx *c, const double *b, double **xp, double **h, int &nh, float **dh, float *ms)
 {
     *ms = 0.f;
     c->r0_ready = false;
     if (c->prm.warm_order == 0) return ST_OK;
+    if (c->prm.warm_order == 3 && c->prm.precision == ST_FP32 && march_ok(c->lv[0]))
+        return warm_mr(c, b, xp, h, nh, dh, ms);
     if (!h[0]) {
         CKR(dalloc(c, &h[0], c->nvec));
         CKR(dalloc(c, &h[1], c->nvec));
@@ -1016,7 +1141,8 @@ static int warm_extrapolate(st_ctx *c, const double *b, double **xp, double **h,
     CK(hipEventCreate(&e));
     CK(hipEventRecord(a, c->s));
     double *x1 = *xp, *x2 = h[0], *f = h[1];  // f: x3, or free
-    const int ord = std::min(nh, c->prm.warm_order);
+    nh = std::min(nh, 2);
+    const int ord = std::min(nh, std::min(c->prm.warm_order, 2));
     bool take = false;
     if (ord >= 1) {
         if (ord == 2) launch_lincomb3(c->nvec, 3.0, x1, -3.0, x2, 1.0, f, c->s);  // 3 x1 - 3 x2 + x3
@@ -1158,7 +1284,10 @@ static int learned_guess(st_ctx *c, float *ms, int *used)
     if (!c->have_x) launch_axpby<double, double>(c->nvec, 1.0, c->XD, 0.0, c->X, c->s);
     int take = 0;
     c->learned_calls++;
-    const bool skip = c->have_x && c->learned_rejects >= 3 && (c->learned_calls % 4) != 0;
+    // back-off: after 3 consecutive rejections the network is tried every 4th call, and
+    // the period doubles with every further rejection up to every 64th
+    const int per = 1 << std::min(std::max(c->learned_rejects - 1, 0), 6);
+    const bool skip = c->have_x && c->learned_rejects >= 3 && (c->learned_calls % per) != 0;
     if ((c->have_x && !c->have_rhop) || skip) {
         take = 0;  // no design for the held solution, or backing off: plain warm start
     } else {
@@ -1167,8 +1296,11 @@ static int learned_guess(st_ctx *c, float *ms, int *used)
             CKR(predict(c, c->rhop, c->xcp));
             launch_axpby<double, double>(nvc, -1.0, c->xcp, 1.0, c->xcn, c->s);  // xcn = net(rho) - net(rho_prev)
         }
+        // the residual of the start without the network: known when the warm start left
+        // it in u_0, else computed
         double r_base = 0.0, r_cand = 0.0;
-        CKR(resid_norm(c, c->B, c->X, &r_base));
+        if (c->r0_ready) r_base = c->r0_beta;
+        else CKR(resid_norm(c, c->B, c->X, &r_base));
         launch_axpby<double, double>(c->nvec, 1.0, c->X, 0.0, c->Wv, c->s);  // keep the base
         prolong_state_add(c, c->xcn, c->X);
         CKR(resid_norm(c, c->B, c->X, &r_cand));
@@ -1201,11 +1333,11 @@ int st_solve(st_ctx *c, int init, const double *x0, double *x_out, st_stats *sta
         c->nxh = 0;  // a new sequence
     } else if (init == ST_INIT_KEEP) {
         if (!c->have_x) init = ST_INIT_ZERO;
-        else CKR(warm_extrapolate(c, c->B, &c->X, c->XH, c->nxh, &wms));
+        else CKR(warm_extrapolate(c, c->B, &c->X, c->XH, c->nxh, c->DX, &wms));
     } else if (init == ST_INIT_LEARNED) {
         float ims = 0.f;
         int used = 0;
-        if (c->have_x) CKR(warm_extrapolate(c, c->B, &c->X, c->XH, c->nxh, &wms));
+        if (c->have_x) CKR(warm_extrapolate(c, c->B, &c->X, c->XH, c->nxh, c->DX, &wms));
         else c->nxh = 0;
         CKR(learned_guess(c, &ims, &used));
         int rc = solve_core(c, c->B, c->X, c->XD, ST_INIT_KEEP, stats);
@@ -1520,7 +1652,7 @@ int st_to_step(st_ctx *c, int init, double volfrac, double move, double eta, dou
     int fmode = (init == ST_INIT_ZERO || !c->have_x) ? ST_INIT_ZERO : ST_INIT_KEEP;
     float ims = 0.f, wms = 0.f;
     int used = 0;
-    if (fmode == ST_INIT_KEEP) CKR(warm_extrapolate(c, c->B, &c->X, c->XH, c->nxh, &wms));
+    if (fmode == ST_INIT_KEEP) CKR(warm_extrapolate(c, c->B, &c->X, c->XH, c->nxh, c->DX, &wms));
     else c->nxh = 0;
     if (init == ST_INIT_LEARNED && c->have_x && c->net && net_num_samples(c->net) > 0)
         CKR(learned_guess(c, &ims, &used));
@@ -1537,7 +1669,7 @@ int st_to_step(st_ctx *c, int init, double volfrac, double move, double eta, dou
     // adjoint: A lam = G x (A symmetric), homogeneous Dirichlet data
     int amode = (init == ST_INIT_ZERO || !c->have_lam) ? ST_INIT_ZERO : ST_INIT_KEEP;
     float lms = 0.f;
-    if (amode == ST_INIT_KEEP) CKR(warm_extrapolate(c, c->BA, &c->LAM, c->LH, c->nlh, &lms));
+    if (amode == ST_INIT_KEEP) CKR(warm_extrapolate(c, c->BA, &c->LAM, c->LH, c->nlh, c->DL, &lms));
     else c->nlh = 0;
     rc = solve_core(c, c->BA, c->LAM, c->ZERO, amode, adj);
     if (adj) {

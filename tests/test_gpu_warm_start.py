@@ -2,9 +2,11 @@
 
 A solve with ST_INIT_KEEP starts from the quadratic (or linear) extrapolation of the
 sequence's previous solutions when its true FP64 residual is below that of the held
-solution.  The contract: the start is never worse than the held solution, the
-solutions and the optimisation's designs do not change beyond the solver tolerance,
-and along an optimisation run the starts are better.
+solution (orders 1, 2), or from the combination of the held solution and the last
+differences of solutions with the least residual (order 3, the FP32 path).  The
+contract: the start is never worse than the held solution, the solutions and the
+optimisation's designs do not change beyond the solver tolerance, and along an
+optimisation run the starts are better.
 """
 import numpy as np
 import pytest
@@ -24,9 +26,10 @@ def _sequence(n, k, seed=7):
     return [(1 - t) * a + t * b for t in np.linspace(0.0, 0.3, k)] + [c]
 
 
-@pytest.mark.parametrize("order", [1, 2])
-def test_extrapolated_start_is_never_worse_than_the_held_solution(make_solver, order):
-    n = 32
+@pytest.mark.parametrize("order,n", [(1, 32), (2, 32), (3, 64)])
+def test_extrapolated_start_is_never_worse_than_the_held_solution(make_solver, order, n):
+    """(order 3 at 64^3: its path needs the tiled finest level; smaller grids fall back
+    to order 2)"""
     P = channel_problem(3, n)
     designs = _sequence(n, 7)
     runs = {}
@@ -48,13 +51,41 @@ def test_extrapolated_start_is_never_worse_than_the_held_solution(make_solver, o
     assert sum(runs[order][k][1]["relres0"] for k in range(3, 7)) < 0.5 * sum(runs[0][k][1]["relres0"] for k in range(3, 7))
 
 
+def test_minimal_residual_start_is_never_worse_than_the_extrapolation(make_solver):
+    """warm_order 3 minimises the residual over the held solution plus the last three
+    differences of solutions, a span that holds both polynomial extrapolations: along the
+    smooth part of a sequence its start's residual is at most the quadratic
+    extrapolation's (up to the solver tolerance by which the held solutions of the two
+    runs differ), and the solutions agree.  64^3: the tiled finest level."""
+    n = 64
+    P = channel_problem(3, n)
+    designs = _sequence(n, 8)
+    runs = {}
+    for wo in (2, 3):
+        s = solver_for(make_solver, P, None, precision="fp32", rtol=1e-8, warm_order=wo)
+        out = []
+        for k, d in enumerate(designs):
+            s.set_design(d)
+            x, st = s.solve(init="zero" if k == 0 else "keep")
+            assert st["converged"] and st["relres"] <= 1e-8
+            out.append((x, st))
+        runs[wo] = out
+    for k in range(3, len(designs) - 1):
+        (x2, s2), (x3, s3) = runs[2][k], runs[3][k]
+        assert s3["relres0"] <= 1.05 * s2["relres0"] + 1e-8, (k, s3["relres0"], s2["relres0"])
+        assert rel(x3, x2) < 1e-6
+    # the jump to an unrelated design: never worse than the held solution either
+    assert runs[3][-1][1]["relres0"] <= runs[2][-1][1]["relres0"] * 1.05 + 1e-8
+
+
 def test_optimisation_run_with_extrapolated_starts(make_solver):
-    """A 32^3 OC run (24 iterations, forward and adjoint warm-started): the same
-    objectives and final design as with the held solution as start, fewer iterations."""
-    n, vf, its = 32, 0.25, 24
+    """A 64^3 OC run (24 iterations, forward and adjoint warm-started): the same
+    objectives and final design as with the held solution as start, fewer iterations
+    with the quadratic extrapolation and fewer still with the minimal-residual start."""
+    n, vf, its = 64, 0.25, 24
     P = channel_problem(3, n)
     res = {}
-    for wo in (0, 2):
+    for wo in (0, 2, 3):
         s = solver_for(make_solver, P, np.full(n ** 3, vf), precision="fp32", rtol=1e-8, warm_order=wo)
         J, fi, ai = [], 0, 0
         for it in range(its):
@@ -66,7 +97,10 @@ def test_optimisation_run_with_extrapolated_starts(make_solver):
         res[wo] = (np.array(J), s.get_design(), fi, ai)
     J0, d0, f0, a0 = res[0]
     J2, d2, f2, a2 = res[2]
-    print(f"forward iterations {f0} -> {f2}, adjoint {a0} -> {a2}", flush=True)
-    assert np.max(np.abs(J2 - J0) / np.abs(J0)) < 1e-6
-    assert rel(d2, d0) < 1e-4
+    J3, d3, f3, a3 = res[3]
+    print(f"forward iterations {f0} -> {f2} -> {f3}, adjoint {a0} -> {a2} -> {a3}", flush=True)
+    for J, d in ((J2, d2), (J3, d3)):
+        assert np.max(np.abs(J - J0) / np.abs(J0)) < 1e-6
+        assert rel(d, d0) < 1e-4
     assert f2 < f0 and a2 <= a0
+    assert f3 < f2 and a3 <= a2
