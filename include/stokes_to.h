@@ -68,8 +68,8 @@ typedef struct {
                             2 its quadratic extrapolation from the previous solutions of the
                             sequence, taken when its true FP64 residual is lower; 3 (default) the
                             held solution plus the combination of the last three differences of
-                            solutions with the least residual (FP32 precision on a tiled finest
-                            level; elsewhere 2) */
+                            solutions with the least residual (FP32 precision, 3D, grid side a
+                            multiple of 64; elsewhere 2) */
 } st_params;
 
 typedef struct {
