@@ -79,8 +79,9 @@ typedef struct {
     double relres0;      /* true relative residual of the initial guess */
     double ms;           /* wall time of the solve incl. hierarchy setup (hipEvents) */
     double setup_ms;     /* hierarchy setup part (coarsening + coarsest inverse) */
-    double init_ms;      /* initial-guess part (extrapolation, learned predictor inference and
-                            their residual checks) */
+    double init_ms;      /* initial-guess part (extrapolation or minimal-residual start, learned
+                            predictor inference and their residual checks; the minimal-residual
+                            start sets up the hierarchy itself, so there setup_ms is part of it) */
     int learned_used;    /* ST_INIT_LEARNED: 1 if the network's candidate had the smaller true residual */
 } st_stats;
 

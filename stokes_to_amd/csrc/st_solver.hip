@@ -1012,7 +1012,7 @@ static int resid_norm_u0(st_ctx *c, const double *b, const double *x, double *nr
 // taken when its residual is below ||r1||.  The span holds x1 itself and both
 // polynomial extrapolations (2 x1 - x2 = x1 + D_1, 3 x1 - 3 x2 + x3 = x1 + 2 D_1 - D_2),
 // so the start is never worse than any of them.  r1 is the FP64 residual written in FP32
-// into u_0; A D_i (FP64 operator, FP32 vectors) go into the next three basis slots
+// into u_0; A D_i (the FP32 operator of the solve) go into the next three basis slots
 // (free until the solve); d from the normal equations of their Gram matrix (one pass);
 // the update writes x0 and r0 = r1 - sum d_i A D_i into u_0, which the solve takes as its
 // first residual.  The corrections are small next to x1, so the FP32 differences and
@@ -1035,6 +1035,17 @@ static int warm_mr(st_ctx *c, const double *b, double **xp, double **h, int &nh,
     CK(hipEventCreate(&a));
     CK(hipEventCreate(&e));
     CK(hipEventRecord(a, c->s));
+    // the products below use the FP32 operator of the new design: set up the hierarchy
+    // the solve then uses (its time counts in the start's)
+    if (c->dirty) {
+        const int rc = setup_hierarchy(c);
+        if (rc != ST_OK) {
+            hipEventDestroy(a);
+            hipEventDestroy(e);
+            nh = 0;
+            return rc;
+        }
+    }
     double *x1 = *xp, *f = h[1];
     if (nh >= 1) {  // newest difference x1 - x_2 into the oldest slot, then rotate
         float *t = dh[2];
@@ -1051,7 +1062,7 @@ static int warm_mr(st_ctx *c, const double *b, double **xp, double **h, int &nh,
     bool take = false;
     if (nd >= 1 && std::isfinite(r1) && r1 > 0.0) {
         float *y = u0 + nv;  // basis slots 1..nd
-        for (int i = 0; i < nd; i++) march_apply_f32(g, c->fl[0], c->h64.al[0], dh[i], y + i * nv, c->s);
+        for (int i = 0; i < nd; i++) march_apply<float>(g, c->fl[0], c->h32.al[0], dh[i], y + i * nv, c->s);
         const int nq = nd * (nd + 1) / 2 + nd;
         const int nbk = (int)mr_blocks(nv);
         launch_mr_gram(nd, nv, y, nv, u0, c->part, c->s);
