@@ -12,7 +12,7 @@ import numpy as np
 import pytest
 
 from conftest import rel
-from helpers import setup, solver_for
+from helpers import oracle_desc, setup, solver_for
 from stokes_to_amd.problems import channel_problem
 
 pytestmark = pytest.mark.gpu
@@ -76,6 +76,42 @@ def test_minimal_residual_start_is_never_worse_than_the_extrapolation(make_solve
         assert rel(x3, x2) < 1e-6
     # the jump to an unrelated design: never worse than the held solution either
     assert runs[3][-1][1]["relres0"] <= runs[2][-1][1]["relres0"] * 1.05 + 1e-8
+
+
+def test_minimal_residual_start_is_the_least_squares_optimum(make_solver, orc):
+    """The start of order 3 against an independent restatement: with the solutions the
+    GPU returned, the oracle's FP64 operator and right-hand side of the new design, and
+    numpy's least squares over x1 + span{x1 - x2, x2 - x3, x3 - x4}, the optimal residual
+    equals the solve's initial residual (relres0) to the FP32 rounding of the
+    differences and their products (bar 1e-3 relative)."""
+    n = 64
+    P = channel_problem(3, n)
+    designs = _sequence(n, 7)[:-1]  # the smooth part
+    s = solver_for(make_solver, P, None, precision="fp32", rtol=1e-8, warm_order=3)
+    xs = []
+    checked = 0
+    for k, d in enumerate(designs):
+        s.set_design(d)
+        x, st = s.solve(init="zero" if k == 0 else "keep")
+        assert st["converged"]
+        if k >= 4:
+            fixed, vol, pen = oracle_desc(orc, P, d)
+            A = lambda v: orc.apply(3, P.shape, fixed, vol, pen, v)
+            b = orc.lift_rhs(3, P.shape, fixed, P.dirichlet)
+            x1 = xs[-1]
+            D = [xs[-1] - xs[-2], xs[-2] - xs[-3], xs[-3] - xs[-4]]
+            r1 = b - A(x1)
+            Y = np.stack([A(v) for v in D], axis=1)
+            c, *_ = np.linalg.lstsq(Y, r1, rcond=None)
+            r_opt = np.linalg.norm(r1 - Y @ c) / np.linalg.norm(b)
+            r_held = np.linalg.norm(r1) / np.linalg.norm(b)
+            print(f"step {k}: relres0 {st['relres0']:.6e}, least-squares optimum {r_opt:.6e}, "
+                  f"held solution {r_held:.3e}", flush=True)
+            assert r_opt < 0.9 * r_held
+            assert abs(st["relres0"] - r_opt) <= 1e-3 * r_opt
+            checked += 1
+        xs.append(x)
+    assert checked == 3
 
 
 def test_optimisation_run_with_extrapolated_starts(make_solver):
