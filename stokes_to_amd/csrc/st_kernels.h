@@ -108,7 +108,6 @@ unsigned mr_blocks(long n);
 void launch_mr_gram(int nd, long n, const float *y, long ld, const float *r, double *part, hipStream_t);
 void launch_mr_update(int nd, long n, const double *x1, const float *const *D, const float *y, long ld,
                       const MrCoef &cf, double *x0, float *r, double *part, hipStream_t);
-void march_apply_f32(const Lvl &, const uint8_t *, const double *al, const float *x, float *y, hipStream_t);
 template <typename TS> void launch_fill(long n, TS *y, double v, hipStream_t);
 
 // 2.5D marching kernels (st_march.hip) for 3D levels with ny % 4 == 0, nz % 64 == 0

@@ -523,12 +523,6 @@ void march_apply_mixed(const Lvl &g, const uint8_t *fl, const double *al, const 
     launch_march<double, M_APPLY>(g, fl, al, (const double *)nullptr, x, y, (double *)nullptr, 0, 0.f, s);
 }
 
-// FP64 operator (FP64 alpha and arithmetic) on an FP32 vector, FP32 result
-void march_apply_f32(const Lvl &g, const uint8_t *fl, const double *al, const float *x, float *y, hipStream_t s)
-{
-    launch_march<double, M_APPLY>(g, fl, al, (const double *)nullptr, x, y, (double *)nullptr, 0, 0.f, s);
-}
-
 #define INST_M(T)                                                                                              \
     template void march_apply<T>(const Lvl &, const uint8_t *, const T *, const T *, T *, hipStream_t);       \
     template void march_residual<T>(const Lvl &, const uint8_t *, const T *, const T *, const T *, T *,       \
