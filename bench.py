@@ -64,8 +64,8 @@ def parse():
     ap.add_argument("--mean", type=float, default=0.2, help="GRF mean (grf workload)")
     ap.add_argument("--rtol", type=float, default=1e-8)
     ap.add_argument("--mp", default="fp32", choices=["fp32", "bf16"])
-    ap.add_argument("--nu", type=int, nargs=2, default=(2, 2), metavar=("PRE", "POST"),
-                    help="smoothing sweeps of the V-cycle (all contexts)")
+    ap.add_argument("--nu", type=int, nargs=2, default=None, metavar=("PRE", "POST"),
+                    help="smoothing sweeps of the V-cycle (all contexts; default: the library's)")
     ap.add_argument("--warm-order", type=int, default=None, choices=[0, 1, 2, 3],
                     help="start of a sequence of solves (st_params.warm_order; default: the library's)")
     ap.add_argument("--epochs", type=int, default=3)
@@ -83,10 +83,22 @@ def log(args, *a):
         print(*a, file=sys.stderr, flush=True)
 
 
+def sweeps(args):
+    """(nu_pre, nu_post) of every context: --nu, else the library's default."""
+    if args.nu is not None:
+        return tuple(args.nu)
+    import ctypes as ct
+    from stokes_to_amd import _lib
+    p = _lib.StParams()
+    _lib.load().st_default_params(ct.byref(p), 3, args.n)
+    return (p.nu_pre, p.nu_post)
+
+
 def new_ctx(args, prob, precision):
     from stokes_to_amd.steady import FluidSteadyState
     kw = {} if args.warm_order is None else {"warm_order": args.warm_order}
-    s = FluidSteadyState(3, args.n, precision=precision, rtol=args.rtol, nu_pre=args.nu[0], nu_post=args.nu[1], **kw)
+    nu = sweeps(args)
+    s = FluidSteadyState(3, args.n, precision=precision, rtol=args.rtol, nu_pre=nu[0], nu_post=nu[1], **kw)
     s.init_boundary(prob.fixed)
     s.update_b(dirichlet=prob.dirichlet)
     return s
@@ -357,7 +369,7 @@ def host_cores():
 
 
 def cpu_baseline(args, prob, rho, gpu_iters):
-    """The oracle (the CPU FP64 restatement of the same FGMRES + V(2,2), OpenMP) on the
+    """The oracle (the CPU FP64 restatement of the same FGMRES + V-cycle, OpenMP) on the
     host cores.  Headline: hierarchy setup + --cpu-iters FGMRES iterations of the same
     256^3 system, scaled to a whole solve with the iteration count of the FP64 GPU
     solve.  Beside it, whole CPU runs of the smaller configurations: the 2D 64^2 TO run
@@ -377,7 +389,8 @@ def cpu_baseline(args, prob, rho, gpu_iters):
 
     pen, b = system(prob, rho)
     t0 = time.perf_counter()
-    mg = oracle.MG(3, prob.shape, prob.fixed, pen, nu_pre=2, nu_post=2, omega=1.0, tang=0)
+    nu = sweeps(args)
+    mg = oracle.MG(3, prob.shape, prob.fixed, pen, nu_pre=nu[0], nu_post=nu[1], omega=1.0, tang=0)
     t1 = time.perf_counter()
     k = max(1, args.cpu_iters)
     _, it, rr, _ = mg.fgmres(b, x0=prob.dirichlet.copy(), rtol=1e-30, restart=k, maxit=k)
@@ -392,7 +405,7 @@ def cpu_baseline(args, prob, rho, gpu_iters):
         P = channel_problem(3, n)
         pen_n, b_n = system(P, grf_density(3, n, mean=0.3, sigma=8.0 * n / 64, seed=12345))
         t0 = time.perf_counter()
-        m = oracle.MG(3, P.shape, P.fixed, pen_n, nu_pre=2, nu_post=2, omega=1.0, tang=0)
+        m = oracle.MG(3, P.shape, P.fixed, pen_n, nu_pre=nu[0], nu_post=nu[1], omega=1.0, tang=0)
         _, itn, rrn, _ = m.fgmres(b_n, x0=P.dirichlet.copy(), rtol=1e-8)
         small[f"3d_{n}_grf_solve_ms"] = round((time.perf_counter() - t0) * 1e3, 1)
         small[f"3d_{n}_grf_iters"] = int(itn)
@@ -408,7 +421,7 @@ def cpu_baseline(args, prob, rho, gpu_iters):
     return {
         "value": round(est * 1e3, 1), "unit": "ms/solve", "cores": threads, "kind": "port",
         "cpu_model": model, "host_cpu_count": ncpu,
-        "sample": f"oracle FGMRES+V(2,2) FP64, OpenMP on {threads} threads (the cores this process may use), on "
+        "sample": f"oracle FGMRES+V({nu[0]},{nu[1]}) FP64, OpenMP on {threads} threads (the cores this process may use), on "
                   f"the same {args.n}^3 system (the plain run's last design): hierarchy setup ({setup_s:.2f} s) "
                   f"+ {it} FGMRES iterations ({per_it:.2f} s each), scaled to the {gpu_iters} iterations of the "
                   f"FP64 GPU solve",

@@ -86,7 +86,7 @@ typedef struct {
 } st_stats;
 
 /* defaults: the configs' material model (alpha_max 1e4, q 0.1, power 1, scale 1/n^2),
-   FP64, rtol 1e-8, FGMRES(30), V(2,2), omega 1, filter radius 1.5, colour passes */
+   FP64, rtol 1e-8, FGMRES(30), V(1,2), omega 1, filter radius 1.5, colour passes */
 void st_default_params(st_params *p, int dim, int n);
 
 /* replaces FluidSteadyState3D::init (ST/FluidSteadyState3D.cpp:12-90): builds the

@@ -659,7 +659,7 @@ void st_default_params(st_params *p, int dim, int n)
     p->rtol = 1e-8;
     p->max_iter = 200;
     p->restart = 30;
-    p->nu_pre = 2;
+    p->nu_pre = 1;
     p->nu_post = 2;
     p->omega = 1.0;
     p->closed = 0;

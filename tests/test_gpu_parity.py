@@ -116,14 +116,16 @@ def test_mg_components_match_oracle(make_solver, orc, dim, n):
             assert rel(s.mg_prolong(l, ec, xin), xo) < 1e-13
 
 
+@pytest.mark.parametrize("nu", [(1, 2), (2, 2)])
 @pytest.mark.parametrize("dim,n", [(3, 16), (2, 32), (3, 64)])
-def test_vcycle_matches_oracle(make_solver, orc, dim, n):
+def test_vcycle_matches_oracle(make_solver, orc, dim, n, nu):
     """At 64^3 the fine level runs the tiled kernels, including the smoother pass from
-    x = 0 that starts the V-cycle."""
+    x = 0 that starts the V-cycle.  V(1,2) is the default, V(2,2) the other sweep
+    counts' representative."""
     P, rho = setup(dim, n)
-    s = solver_for(make_solver, P, rho)
+    s = solver_for(make_solver, P, rho, nu_pre=nu[0], nu_post=nu[1])
     fixed, vol, pen = oracle_desc(orc, P, rho)
-    mg = orc.MG(dim, P.shape, fixed, pen, nu_pre=2, nu_post=2, omega=1.0, tang=0)
+    mg = orc.MG(dim, P.shape, fixed, pen, nu_pre=nu[0], nu_post=nu[1], omega=1.0, tang=0)
     b = orc.lift_rhs(dim, P.shape, fixed, P.dirichlet)
     z = s.mg_vcycle(b)
     zo = mg.vcycle(b)

@@ -20,8 +20,8 @@ pytestmark = pytest.mark.gpu
 @pytest.mark.parametrize("prec", ["fp32", "bf16"])
 def test_vcycle_with_one_pass_sweeps_matches_colour_passes(make_solver, orc, n, prec):
     P, rho = setup(3, n)
-    a = solver_for(make_solver, P, rho, precision=prec, fused_sweep=1)
-    b = solver_for(make_solver, P, rho, precision=prec, fused_sweep=0)
+    a = solver_for(make_solver, P, rho, precision=prec, fused_sweep=1, nu_pre=2, nu_post=2)
+    b = solver_for(make_solver, P, rho, precision=prec, fused_sweep=0, nu_pre=2, nu_post=2)
     rng = np.random.default_rng(n)
     v = rng.standard_normal(P.fixed.size)
     za, zb = a.mg_vcycle(v), b.mg_vcycle(v)
