@@ -1059,7 +1059,7 @@ static int warm_mr(st_ctx *c, const double *b, double **xp, double **h, int &nh,
     double r1 = 0.0;
     CKR(resid_norm_u0(c, b, x1, &r1));
     double r0 = r1;
-    bool take = false;
+    bool take = false, updated = false;  // updated: u_0 holds r0, no longer r1
     if (nd >= 1 && std::isfinite(r1) && r1 > 0.0) {
         float *y = u0 + nv;  // basis slots 1..nd
         for (int i = 0; i < nd; i++) march_apply<float>(g, c->fl[0], c->h32.al[0], dh[i], y + i * nv, c->s);
@@ -1105,6 +1105,7 @@ static int warm_mr(st_ctx *c, const double *b, double **xp, double **h, int &nh,
         // predicted ||r0||^2 = ||r1||^2 - g.d (d solves the normal equations)
         if (ok && gd > 1e-6 * r1 * r1) {
             launch_mr_update(nd, nv, x1, dh, y, nv, cf, f, u0, c->part, c->s);
+            updated = true;
             launch_sum(nbk, 1, c->part, c->red, c->s);
             double rr = 0.0;
             CK(hipMemcpyAsync(&rr, c->red, sizeof(double), hipMemcpyDeviceToHost, c->s));
@@ -1124,7 +1125,7 @@ static int warm_mr(st_ctx *c, const double *b, double **xp, double **h, int &nh,
         launch_axpby<double, double>(nv, 1.0, x1, 0.0, f, c->s);
         h[1] = h[0];
         h[0] = f;
-        c->r0_ready = (r0 == r1);
+        c->r0_ready = !updated;
         c->r0_beta = r1;
     }
     nh = std::min(nh + 1, 3);
