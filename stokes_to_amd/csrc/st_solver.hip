@@ -806,6 +806,8 @@ int st_get_params(const st_ctx *c, st_params *p)
 long st_num_dofs(const st_ctx *c) { return c ? c->ndof : -1; }
 long st_num_cells(const st_ctx *c) { return c ? c->ncell : -1; }
 
+static void design_to_alpha(st_ctx *c, const double *rho_dev);
+
 int st_set_geometry(st_ctx *c, const uint8_t *fixed, int closed)
 {
     if (!c || !fixed || closed < -1 || closed > 1) return ST_EINVAL;
@@ -819,6 +821,11 @@ int st_set_geometry(st_ctx *c, const uint8_t *fixed, int closed)
     c->geom = true;
     c->dirty = true;
     c->nxh = c->nlh = 0;
+    // a penalty from a design follows the new free faces
+    if (c->rho_used) {
+        design_to_alpha(c, c->rho_used);
+        CK(hipStreamSynchronize(c->s));
+    }
     return ST_OK;
 }
 

@@ -247,6 +247,29 @@ def test_solve_is_bitwise_reproducible(make_solver, prec):
 
 
 
+def test_design_penalty_follows_a_new_geometry(make_solver):
+    """A penalty set from a design is recomputed when st_set_geometry changes the free
+    faces afterwards: faces freed by the new geometry get their penalty (a design set
+    first and the geometry changed later solves bit for bit as the geometry set first)."""
+    n = 16
+    P, rho = setup(3, n)
+    nc = n ** 3
+    idx = nc + np.flatnonzero(P.fixed[nc:] == 0)[::5]  # free faces of the channel
+    more = P.fixed.copy()
+    more[idx] = 1
+    a = make_solver(3, n, rtol=1e-10)
+    a.init_boundary(more)
+    a.set_design(rho)
+    a.init_boundary(P.fixed)  # frees idx again
+    a.update_b(dirichlet=P.dirichlet)
+    b = solver_for(make_solver, P, rho, rtol=1e-10)
+    xa, sa = a.solve()
+    xb, sb = b.solve()
+    assert sa["converged"] and sb["converged"]
+    assert sa["iters"] == sb["iters"]
+    assert np.array_equal(xa, xb)
+
+
 @pytest.mark.parametrize("dim,n,prec,tol", [(3, 16, "fp64", 1e-12), (2, 32, "fp64", 1e-12), (3, 64, "fp64", 1e-12),
                                             (3, 64, "fp32", 1e-6)])
 def test_smoother_with_fixed_cells_matches_oracle(make_solver, orc, dim, n, prec, tol):
