@@ -32,6 +32,8 @@ def bf16(a):
 @pytest.mark.parametrize("R,ntap,cin,cout,relu", [
     (8, 27, 16, 16, 1), (8, 27, 16, 32, 0), (4, 27, 32, 64, 1), (4, 27, 64, 64, 1),
     (8, 27, 96, 32, 1), (8, 27, 48, 16, 1), (6, 27, 32, 32, 0), (8, 1, 16, 16, 0), (16, 27, 16, 32, 1),
+    # grids where four 16-voxel blocks share each weight load (>= 512 workgroups)
+    (64, 27, 16, 16, 1), (32, 27, 32, 64, 0),
 ])
 def test_conv_layer_matches_oracle(orc, R, ntap, cin, cout, relu):
     rng = np.random.default_rng(R * 1000 + cin + cout)

@@ -82,6 +82,18 @@ def test_oc_matches_oracle(make_solver, orc, dim, n):
     r2, c2 = orc.oc_update(dim, P.shape, 1.5, rho, dc, 0.3)
     assert np.max(np.abs(r1 - r2)) < 1e-9
     assert abs(c1 - c2) < 1e-9
+    # later updates start their multiplier search around the previous multiplier: the
+    # same result as the oracle's bisection from its full bracket, whether the new
+    # multiplier lies inside that start (a similar step) or far outside it (a scaled
+    # sensitivity moves the multiplier by 1e4)
+    rng = np.random.default_rng(3)
+    for scale in (1.0, 1e4, 1e-4):
+        rho_k = np.clip(r1 + 0.01 * rng.standard_normal(rho.size), 1e-3, 1.0)
+        dc_k = dc * (1.0 + 0.1 * rng.random(rho.size)) * scale
+        ra, ca = s.oc_update(rho_k, dc_k, 0.3)
+        rb, cb = orc.oc_update(dim, P.shape, 1.5, rho_k, dc_k, 0.3)
+        assert np.max(np.abs(ra - rb)) < 1e-9, scale
+        assert abs(ca - cb) < 1e-9, scale
 
 
 @pytest.mark.parametrize("dim,n", [(3, 16), (2, 32)])
