@@ -13,7 +13,7 @@ level-L predictor can save, whatever its training:
     minimal-residual coefficient on it is ~0;
   * warm start inside a design sequence (the previous design's solution at hand):
     the exact level-2 *change* saves nothing.
-The measured numbers are printed; DESIGN.md quotes them.
+The measured numbers are printed.
 """
 import numpy as np
 
