@@ -78,6 +78,28 @@ def test_minimal_residual_start_is_never_worse_than_the_extrapolation(make_solve
     assert runs[3][-1][1]["relres0"] <= runs[2][-1][1]["relres0"] * 1.05 + 1e-8
 
 
+@pytest.mark.parametrize("prec", ["fp64", "bf16"])
+def test_order_3_on_other_paths_is_the_quadratic_extrapolation(make_solver, prec):
+    """The minimal-residual start belongs to the FP32 path; on the FP64 and BF16 paths
+    order 3 is order 2: the same starts and solutions, bit for bit."""
+    n = 64
+    P = channel_problem(3, n)
+    designs = _sequence(n, 5)
+    runs = {}
+    for wo in (2, 3):
+        s = solver_for(make_solver, P, None, precision=prec, rtol=1e-8, warm_order=wo)
+        out = []
+        for k, d in enumerate(designs):
+            s.set_design(d)
+            x, st = s.solve(init="zero" if k == 0 else "keep")
+            assert st["converged"]
+            out.append((x, st))
+        runs[wo] = out
+    for (x2, s2), (x3, s3) in zip(runs[2], runs[3]):
+        assert s2["relres0"] == s3["relres0"] and s2["iters"] == s3["iters"]
+        assert np.array_equal(x2, x3)
+
+
 def test_minimal_residual_start_is_the_least_squares_optimum(make_solver, orc):
     """The start of order 3 against an independent restatement: with the solutions the
     GPU returned, the oracle's FP64 operator and right-hand side of the new design, and
