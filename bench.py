@@ -176,9 +176,8 @@ def run_to(args, prob):
     and the profiled kernel timings (taken after the runs, on contexts not timed)."""
     import torch
     # the learned context is created before the warm-start one: the two do the same
-    # solver work; created in the order plain, warm start, learned the learned context
-    # measured 5.7% faster than the warm-start one for identical iterations, created in
-    # this order the two agree within 1%
+    # solver work, and the context created last runs it a few per cent faster, so this
+    # order gives the learned start no placement advantage
     P = new_ctx(args, prob, "fp64")
     L = new_ctx(args, prob, args.mp)
     K = new_ctx(args, prob, args.mp)

@@ -739,6 +739,17 @@ int st_create(st_ctx **out, const st_params *p)
     if (rc == ST_OK) rc = dalloc(c, &c->Wv, c->nvec);
     if (rc == ST_OK) rc = dalloc(c, &c->V, c->nvec * (mr + 1));
     if (rc == ST_OK) rc = dalloc(c, &c->Zb, c->nvec * mr);
+    // the held solutions of the warm starts (and the FP32 differences of the
+    // minimal-residual start) with the context's other vectors, not during a run
+    if (p->warm_order > 0)
+        for (double **pp : {&c->XH[0], &c->XH[1], &c->LH[0], &c->LH[1]})
+            if (rc == ST_OK) rc = dalloc(c, pp, c->nvec);
+    if (p->warm_order == 3 && p->precision == ST_FP32 && march_ok(c->lv[0]))
+        for (float **pp : {&c->DX[0], &c->DX[1], &c->DX[2], &c->DL[0], &c->DL[1], &c->DL[2]}) {
+            double *t = nullptr;
+            if (rc == ST_OK) rc = dalloc(c, &t, (c->nvec + 1) / 2);
+            *pp = reinterpret_cast<float *>(t);
+        }
     c->npart = 8192 * 33;
     if (rc == ST_OK) rc = dalloc(c, &c->part, c->npart);
     if (rc == ST_OK) rc = dalloc(c, &c->red, 64);
@@ -1267,9 +1278,19 @@ static int resid_norm(st_ctx *c, const double *b, const double *x, double *nrm)
 }
 
 // after a forward solve: remember the design its solution belongs to
+// back-off of the learned candidate: after 3 consecutive rejections the network is tried
+// every 4th call, the period doubling with every further rejection up to every 64th
+static bool learned_backs_off(const st_ctx *c, long call)
+{
+    const int per = 1 << std::min(std::max(c->learned_rejects - 1, 0), 6);
+    return c->learned_rejects >= 3 && (call % per) != 0;
+}
+
 static void remember_design(st_ctx *c)
 {
-    if (c->net && c->rhop && c->rho_used) {
+    // kept only when the next learned start will try the network (it needs the design
+    // the held solution belongs to)
+    if (c->net && c->rhop && c->rho_used && !learned_backs_off(c, c->learned_calls + 1)) {
         launch_axpby<double, double>(c->ncell, 1.0, c->rho_used, 0.0, c->rhop, c->s);
         c->have_rhop = true;
     } else {
@@ -1295,6 +1316,13 @@ static int learned_guess(st_ctx *c, float *ms, int *used)
     if (c->dirty) CKR(setup_hierarchy(c));
     const Lvl &gc = c->lv[c->lc];
     const long nvc = (long)gc.nfield * gc.S;
+    c->learned_calls++;
+    if (c->have_x && (!c->have_rhop || learned_backs_off(c, c->learned_calls))) {
+        // no design for the held solution, or backing off: the start without the network
+        if (ms) *ms = 0.f;
+        if (used) *used = 0;
+        return ST_OK;
+    }
     hipEvent_t a, b;
     CK(hipEventCreate(&a));
     CK(hipEventCreate(&b));
@@ -1302,14 +1330,7 @@ static int learned_guess(st_ctx *c, float *ms, int *used)
     const bool corr = c->have_x && c->have_rhop;
     if (!c->have_x) launch_axpby<double, double>(c->nvec, 1.0, c->XD, 0.0, c->X, c->s);
     int take = 0;
-    c->learned_calls++;
-    // back-off: after 3 consecutive rejections the network is tried every 4th call, and
-    // the period doubles with every further rejection up to every 64th
-    const int per = 1 << std::min(std::max(c->learned_rejects - 1, 0), 6);
-    const bool skip = c->have_x && c->learned_rejects >= 3 && (c->learned_calls % per) != 0;
-    if ((c->have_x && !c->have_rhop) || skip) {
-        take = 0;  // no design for the held solution, or backing off: plain warm start
-    } else {
+    {
         CKR(predict(c, c->rho_used, c->xcn));
         if (corr) {
             CKR(predict(c, c->rhop, c->xcp));
