@@ -97,6 +97,9 @@ struct st_ctx {
     double oc_lambda = 0.0;
     // dv = dV/drho = H^T(1/(n W)) depends on the grid and the filter only: formed once
     bool dv_ready = false;
+    // ||B||^2 of the forward right-hand side (it changes only with st_set_rhs / geometry)
+    double bn2 = 0.0;
+    bool bn2_ok = false;
     // kernel timing: event pairs around finest-level launches
     bool prof = false;
     std::vector<hipEvent_t> evpool[ST_PROF_NKIND];
@@ -495,7 +498,15 @@ template <typename TV, typename TW> static int fgmres(st_ctx *c, const double *b
     const int m = c->prm.restart;
     const double rtol = c->prm.rtol;
     double bn2 = 0.0;
-    CKR(dot64(c, b, b, nv, &bn2));
+    if (b == c->B && c->bn2_ok) {
+        bn2 = c->bn2;
+    } else {
+        CKR(dot64(c, b, b, nv, &bn2));
+        if (b == c->B) {
+            c->bn2 = bn2;
+            c->bn2_ok = true;
+        }
+    }
     double bn = std::sqrt(bn2);
     if (!std::isfinite(bn)) return fail(c, ST_ENAN, "non-finite right-hand side");
     if (bn == 0.0) bn = 1.0;
@@ -831,6 +842,7 @@ int st_set_geometry(st_ctx *c, const uint8_t *fixed, int closed)
     if (closed >= 0) c->prm.closed = closed;
     c->geom = true;
     c->dirty = true;
+    c->bn2_ok = false;
     c->nxh = c->nlh = 0;
     // a penalty from a design follows the new free faces
     if (c->rho_used) {
@@ -968,6 +980,7 @@ int st_set_rhs(st_ctx *c, const double *body, const double *dirichlet)
     }
     launch_lift(g, c->fl[0], c->XD, c->BODY, c->B, c->s);
     CK(hipStreamSynchronize(c->s));
+    c->bn2_ok = false;
     c->rhs = true;
     c->nxh = c->nlh = 0;
     return ST_OK;
