@@ -1026,13 +1026,6 @@ static int solve_core_(st_ctx *c, const double *b, double *x, const double *xd, 
 
 // ------------------------------------------------- extrapolated start
 
-// Start of a solve that continues a sequence (ST_INIT_KEEP, ST_INIT_LEARNED): *xp holds
-// the previous solution x1, h[0] = x2 and h[1] = x3 the ones before it (nh valid).  The
-// candidate is the polynomial extrapolation of order min(nh, warm_order) through them,
-// 2 x1 - x2 or 3 x1 - 3 x2 + x3 (the designs of an optimisation change smoothly, and so
-// do their solutions), kept when its true FP64 residual is below that of x1.  After the
-// call *xp is the start and h[0] = x1, h[1] = x2: the three buffers are rotated; x1 is
-// copied only when the candidate loses (or when there is nothing to extrapolate yet).
 static int resid_norm(st_ctx *c, const double *b, const double *x, double *nrm);
 static int resid_norm_u0(st_ctx *c, const double *b, const double *x, double *nrm);
 
@@ -1168,6 +1161,14 @@ static int warm_mr(st_ctx *c, const double *b, double **xp, double **h, int &nh,
     return ST_OK;
 }
 
+// Start of a solve that continues a sequence (ST_INIT_KEEP, ST_INIT_LEARNED; order 3 on
+// the FP32 path is warm_mr above): *xp holds the previous solution x1, h[0] = x2 and
+// h[1] = x3 the ones before it (nh valid).  The candidate is the polynomial
+// extrapolation of order min(nh, warm_order, 2) through them,
+// 2 x1 - x2 or 3 x1 - 3 x2 + x3 (the designs of an optimisation change smoothly, and so
+// do their solutions), kept when its true FP64 residual is below that of x1.  After the
+// call *xp is the start and h[0] = x1, h[1] = x2: the three buffers are rotated; x1 is
+// copied only when the candidate loses (or when there is nothing to extrapolate yet).
 static int warm_extrapolate(st_ctx *c, const double *b, double **xp, double **h, int &nh, float **dh, float *ms)
 {
     *ms = 0.f;
